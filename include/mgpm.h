@@ -1,0 +1,185 @@
+/*
+ * mgpm.h — C ABI of the MI355X-native MERCURY counting path.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every function returns
+ * MGPM_OK (0) or a negative MGPM_ERR_* code; nothing here calls exit() or
+ * throws across the boundary.  The caller owns every handle it receives and
+ * frees it with the matching *_free / *_destroy call.  One handle is used by
+ * one host thread at a time.  Counts are uint64 (exact, wrapping mod 2^64
+ * exactly where the reference's unsigned long long accumulators wrap).
+ *
+ * Streams are passed as `void*` holding a hipStream_t (NULL = the default
+ * stream of the device the handle lives on).
+ *
+ * Reference interfaces replaced (reference file:line):
+ *   main.cc:5-20                         -> the `tc_challenge` CLI over mgpm_tc_count
+ *   tc_challenge.cu:11  TCSolver         -> mgpm_dgraph_upload + mgpm_tc_count
+ *   subgraph_matching/main_mpi.cpp:5-38  -> the `sm` CLI over mgpm_pattern_count
+ *   subgraph_matching/sm_multigpu.cu:97  SMSolverMultiGPU -> mgpm_pattern_count
+ *   graph.cc:6-92  Graph::Graph + custom_alloc.h:33-44 read_file -> mgpm_graph_load
+ *   graph.cc:196-321 orientation_with_division -> mgpm_graph_orientation
+ *   graph_gpu.h:87-188 GraphGPU::init / init_with_division -> mgpm_dgraph_upload
+ *   cutil_subset.h:4-12 CUDA_SAFE_CALL (exit on error) -> negative status codes
+ */
+#ifndef MGPM_H
+#define MGPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPM_ABI_VERSION 1
+
+/* ---- status codes --------------------------------------------------- */
+#define MGPM_OK 0
+#define MGPM_ERR_IO (-1)          /* a file could not be opened / was short */
+#define MGPM_ERR_FORMAT (-2)      /* graph.meta.txt malformed or sizes wrong */
+#define MGPM_ERR_INVALID (-3)     /* bad argument */
+#define MGPM_ERR_NOMEM (-4)       /* host or device allocation failed */
+#define MGPM_ERR_HIP (-5)         /* a HIP runtime call failed */
+#define MGPM_ERR_NODEVICE (-6)    /* no usable GPU / bad device index */
+#define MGPM_ERR_GRAPH (-7)       /* CSR is not simple/sorted/symmetric */
+#define MGPM_ERR_PATTERN (-8)     /* unknown pattern id ("Not support.") */
+#define MGPM_ERR_CAPACITY (-9)    /* request exceeds a capacity/budget */
+#define MGPM_ERR_BUSY (-10)       /* resource in use */
+#define MGPM_ERR_NOTFOUND (-11)   /* unknown id / ticket */
+#define MGPM_ERR_TIMEOUT (-12)    /* wait timed out */
+#define MGPM_ERR_FAILED (-13)     /* request failed (server) */
+#define MGPM_ERR_INTERNAL (-14)
+
+/* ---- patterns -------------------------------------------------------- */
+/* MGPM_TC is triangle counting; 1..8 are the reference's P1..P8
+ * (subgraph_matching/sm_multigpu.cu:148-198). */
+#define MGPM_TC 0
+#define MGPM_P1 1
+#define MGPM_P2 2
+#define MGPM_P3 3
+#define MGPM_P4 4
+#define MGPM_P5 5
+#define MGPM_P6 6
+#define MGPM_P7 7
+#define MGPM_P8 8
+#define MGPM_NUM_PATTERNS 9
+
+typedef struct mgpm_graph mgpm_graph;   /* host CSR, undirected */
+typedef struct mgpm_dgraph mgpm_dgraph; /* device-resident graph */
+typedef struct mgpm_batch mgpm_batch;   /* batched multi-graph launcher */
+typedef struct mgpm_server mgpm_server; /* continuous-batching server */
+
+/* Knobs of the counting kernels.  Pass NULL for the defaults.  In every
+ * field 0 means "default"; for the *_max_deg fields -1 empties that class.
+ * Triangle tasks are classed by the table side's DAG out-degree d+:
+ *   d+ <= small_max_deg  -> 256-slot wave-private LDS tables   (cap 128)
+ *   d+ <= wave_max_deg   -> 2048-slot wave-private LDS tables  (cap 1024)
+ *   d+ <= block_max_deg  -> workgroup-shared LDS tables        (cap 16384)
+ *   larger               -> binary search in the global list. */
+typedef struct mgpm_tune {
+  int32_t small_max_deg;
+  int32_t wave_max_deg;
+  int32_t block_max_deg;
+  int32_t chunk_work;     /* TC: max streamed list items per task */
+  int32_t grid_blocks;    /* cap on persistent workgroups per kernel */
+  int32_t pattern_waves;  /* P1..P8: waves per workgroup */
+  int32_t reserved[2];
+} mgpm_tune;
+
+/* Per-call measurements. */
+typedef struct mgpm_stats {
+  double kernel_ms;   /* device time of the counting kernels: hipEvents
+                         recorded on the launch stream around them */
+  double e2e_ms;      /* host wall time of the whole call */
+  double prep_ms;     /* host/device preprocessing done inside this call
+                         (0 when the plan was already resident) */
+  uint64_t n;         /* vertices */
+  uint64_t m;         /* undirected edges */
+  uint64_t m_plus;    /* DAG arcs (TC) */
+  uint64_t s_plus;    /* sum over DAG arcs u->v of d+(v) (TC wedge checks) */
+  uint64_t probes;    /* TC: hash probes issued by the chosen schedule */
+  uint64_t bytes_alg; /* TC: 8(n+1) + 20 m+ + 4 S+ */
+  int32_t path;       /* implementation path id (diagnostic) */
+  int32_t reserved[3];
+} mgpm_stats;
+
+typedef struct mgpm_graph_info {
+  int32_t n;
+  int32_t max_degree;      /* recomputed from the CSR */
+  int64_t nnz;             /* CSR entries = 2m */
+  int32_t meta_max_degree; /* as stored in graph.meta.txt (0 if not loaded) */
+  int32_t reserved;
+} mgpm_graph_info;
+
+/* ---- errors / version -------------------------------------------------- */
+const char *mgpm_strerror(int code);
+int mgpm_abi_version(void);
+/* Last error detail message of the calling thread ("" if none). */
+const char *mgpm_last_error(void);
+
+/* ---- host graphs ---------------------------------------------------------
+ * Reference: graph.cc:6-92 (Graph::Graph), custom_alloc.h:33-44 (read_file).
+ * Reads <prefix>.meta.txt / .vertex.bin / .edge.bin.  Where the reference
+ * asserts or exits, this returns MGPM_ERR_FORMAT / MGPM_ERR_IO, and it also
+ * rejects a CSR that is not simple, sorted and symmetric (MGPM_ERR_GRAPH),
+ * which the reference silently assumes. */
+int mgpm_graph_load(const char *prefix, mgpm_graph **out);
+/* Copies the arrays.  Validated like mgpm_graph_load. */
+int mgpm_graph_from_csr(int32_t n, int64_t nnz, const int64_t *rowptr,
+                        const int32_t *colidx, mgpm_graph **out);
+/* Writes the reference format (meta: "n 2m 4 8 1 2 maxdeg 0 0 0"). */
+int mgpm_graph_write(const mgpm_graph *g, const char *prefix);
+int mgpm_graph_info_get(const mgpm_graph *g, mgpm_graph_info *info);
+/* Borrowed views of the CSR, valid until mgpm_graph_free. */
+int mgpm_graph_csr(const mgpm_graph *g, const int64_t **rowptr,
+                   const int32_t **colidx);
+int mgpm_graph_free(mgpm_graph *g);
+
+/* Synthetic graphs (no counterpart in the reference, which ships none).
+ * RMAT: 2^scale vertices, edge_factor * 2^scale draws, quadrant
+ * probabilities a, b, c (d = 1-a-b-c), no noise or permutation; ER:
+ * `draws` uniform pairs.  Both symmetrize, drop self-loops and duplicates.
+ * Draw i depends only on (seed, i), so output is independent of threads. */
+int mgpm_graph_generate_rmat(int32_t scale, int32_t edge_factor, double a,
+                             double b, double c, uint64_t seed,
+                             mgpm_graph **out);
+int mgpm_graph_generate_er(int32_t n, int64_t draws, uint64_t seed,
+                           mgpm_graph **out);
+
+/* ---- degree-oriented DAG (host) -------------------------------------------
+ * Reference: graph.cc:196-321 orientation_with_division(deg_thd).  sizes[0]
+ * = DAG arcs, sizes[1] = small arcs, sizes[2] = large arcs; call once with
+ * NULL arrays to get the sizes, then with caller-allocated arrays (row
+ * pointers n+1 int64, column arrays of the reported sizes). */
+int mgpm_graph_orientation(const mgpm_graph *g, int32_t deg_thd,
+                           int64_t sizes[3], int64_t *dag_rowptr,
+                           int32_t *dag_colidx, int64_t *small_rowptr,
+                           int32_t *small_colidx, int64_t *large_rowptr,
+                           int32_t *large_colidx);
+
+/* ---- devices and device graphs -------------------------------------------
+ * Reference: graph_gpu.h:87-188.  Uploads the undirected CSR (P1..P8) and
+ * builds + uploads the triangle-count plan (DAG and table/stream tasks). */
+int mgpm_device_count(int *count);
+int mgpm_device_synchronize(int device);
+int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
+                       mgpm_dgraph **out);
+/* Device bytes held by the handle (admission control uses this). */
+int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes);
+int mgpm_dgraph_free(mgpm_dgraph *dg);
+
+/* ---- counting ----------------------------------------------------------
+ * Reference: tc_challenge.cu:11-118 (TCSolver) and the kernels of
+ * gpu_kernels/tc_challenge.cuh:4-303. */
+int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
+                  uint64_t *count, mgpm_stats *stats);
+/* Reference: subgraph_matching/sm_multigpu.cu:97-239 and the kernels of
+ * subgraph_matching/pattern_kernels.cuh:6-963, counter combination
+ * sm_multigpu.cu:204-233.  pattern in 1..8 (MGPM_TC also accepted). */
+int mgpm_pattern_count(mgpm_dgraph *dg, int pattern, const mgpm_tune *tune,
+                       void *stream, uint64_t *count, mgpm_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGPM_H */

@@ -1,0 +1,404 @@
+// C ABI: host graphs, device graphs and triangle counting.
+#include <omp.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "capi_common.hpp"
+
+namespace mgpm {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return MGPM_ERR_NOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    return MGPM_ERR_NODEVICE;
+  return MGPM_ERR_HIP;
+}
+
+size_t TcResident::bytes() const {
+  size_t b = col.bytes + partners.bytes;
+  for (int c = 0; c < kTcNumClasses; ++c) b += tasks[c].bytes;
+  return b;
+}
+
+TcPlanParams tc_params_from(const mgpm_tune *t) {
+  TcPlanParams p;
+  p.small_max = kTcSmallMaxDeg;
+  p.wave_max = kTcWaveMaxDeg;
+  p.block_max = kTcBlockMaxDeg;
+  if (!t) return p;
+  auto pick = [](int32_t v, int32_t def, int32_t cap) {
+    if (v == 0) return def;
+    if (v < 0) return 0;
+    return std::min(v, cap);
+  };
+  p.small_max = pick(t->small_max_deg, kTcSmallMaxDeg, kTcSmallMaxDeg);
+  p.wave_max = std::max(p.small_max,
+                        pick(t->wave_max_deg, kTcWaveMaxDeg, kTcWaveMaxDeg));
+  p.block_max = std::max(
+      p.wave_max, pick(t->block_max_deg, kTcBlockMaxDeg, kTcBlockMaxDeg));
+  if (t->chunk_work > 0) p.chunk_work = t->chunk_work;
+  return p;
+}
+
+namespace {
+
+bool same_params(const TcPlanParams &a, const TcPlanParams &b) {
+  return a.small_max == b.small_max && a.wave_max == b.wave_max &&
+         a.block_max == b.block_max && a.chunk_work == b.chunk_work;
+}
+
+template <class T>
+int upload_vec(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
+  MGPM_HIP(buf.alloc(v.size() * sizeof(T)));
+  if (!v.empty())
+    MGPM_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T),
+                            hipMemcpyHostToDevice, s));
+  return MGPM_OK;
+}
+
+// (Re)builds the triangle plan on the host and uploads it.
+int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
+                     hipStream_t s) {
+  TcPlan plan;
+  int rc = build_tc_plan(*dg->host, params, plan);
+  if (rc) return fail(rc, "triangle plan: DAG has 2^32 or more arcs");
+  TcResident &r = dg->tc;
+  r.params = params;
+  r.s_plus = plan.s_plus;
+  r.probes = plan.probes;
+  r.m_plus = static_cast<uint64_t>(plan.m_plus);
+  r.max_out = plan.max_out;
+  if ((rc = upload_vec(r.col, plan.dag_col, s))) return rc;
+  if ((rc = upload_vec(r.partners, plan.partners, s))) return rc;
+  for (int c = 0; c < kTcNumClasses; ++c) {
+    if ((rc = upload_vec(r.tasks[c], plan.tasks[c], s))) return rc;
+    r.dev.tasks[c] = r.tasks[c].as<TcTask>();
+    r.dev.ntasks[c] = static_cast<int>(plan.tasks[c].size());
+    r.dev.class_max_tlen[c] = plan.class_max_tlen[c];
+  }
+  r.dev.col = r.col.as<int32_t>();
+  r.dev.ncol = static_cast<uint32_t>(plan.m_plus);
+  r.dev.partners = r.partners.as<TcPartner>();
+  r.dev.queues = dg->scratch.as<unsigned>();
+  r.dev.result =
+      reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 64);
+  // The host vectors die with `plan`: wait for the copies.
+  MGPM_HIP(hipStreamSynchronize(s));
+  return MGPM_OK;
+}
+
+int wrap_graph(std::shared_ptr<CsrGraph> g, mgpm_graph **out) {
+  mgpm_graph *h = new (std::nothrow) mgpm_graph;
+  if (!h) return fail(MGPM_ERR_NOMEM, "out of host memory");
+  h->g = std::move(g);
+  *out = h;
+  return MGPM_OK;
+}
+
+int set_device(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(MGPM_ERR_NODEVICE, "no HIP device visible");
+  if (device < 0 || device >= count)
+    return fail(MGPM_ERR_NODEVICE, "device index out of range");
+  MGPM_HIP(hipSetDevice(device));
+  return MGPM_OK;
+}
+
+}  // namespace
+
+const char *last_error_cstr() { return g_last_error.c_str(); }
+
+}  // namespace mgpm
+
+using namespace mgpm;
+
+#define MGPM_GUARD_BEGIN try {
+#define MGPM_GUARD_END                                            \
+  }                                                               \
+  catch (const std::bad_alloc &) {                                \
+    return fail(MGPM_ERR_NOMEM, "out of host memory");            \
+  }                                                               \
+  catch (const std::exception &ex) {                              \
+    return fail(MGPM_ERR_INTERNAL, ex.what());                    \
+  }                                                               \
+  catch (...) {                                                   \
+    return fail(MGPM_ERR_INTERNAL, "unknown exception");          \
+  }
+
+extern "C" {
+
+const char *mgpm_strerror(int code) {
+  switch (code) {
+    case MGPM_OK: return "ok";
+    case MGPM_ERR_IO: return "i/o error";
+    case MGPM_ERR_FORMAT: return "bad graph format";
+    case MGPM_ERR_INVALID: return "invalid argument";
+    case MGPM_ERR_NOMEM: return "out of memory";
+    case MGPM_ERR_HIP: return "HIP runtime error";
+    case MGPM_ERR_NODEVICE: return "no such device";
+    case MGPM_ERR_GRAPH: return "graph is not simple, sorted and symmetric";
+    case MGPM_ERR_PATTERN: return "Not support.";
+    case MGPM_ERR_CAPACITY: return "capacity exceeded";
+    case MGPM_ERR_BUSY: return "busy";
+    case MGPM_ERR_NOTFOUND: return "not found";
+    case MGPM_ERR_TIMEOUT: return "timed out";
+    case MGPM_ERR_FAILED: return "request failed";
+    case MGPM_ERR_INTERNAL: return "internal error";
+    default: return "unknown error";
+  }
+}
+
+int mgpm_abi_version(void) { return MGPM_ABI_VERSION; }
+
+const char *mgpm_last_error(void) { return last_error_cstr(); }
+
+// ------------------------------------------------------------ host graphs
+
+int mgpm_graph_load(const char *prefix, mgpm_graph **out) {
+  if (!prefix || !out) return fail(MGPM_ERR_INVALID, "null argument");
+  MGPM_GUARD_BEGIN
+  auto g = std::make_shared<CsrGraph>();
+  std::string err;
+  int rc = read_graph(prefix, *g, err);
+  if (rc) return fail(rc, err);
+  return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_from_csr(int32_t n, int64_t nnz, const int64_t *rowptr,
+                        const int32_t *colidx, mgpm_graph **out) {
+  if (!out || n < 0 || nnz < 0 || !rowptr || (nnz > 0 && !colidx))
+    return fail(MGPM_ERR_INVALID, "bad CSR arguments");
+  MGPM_GUARD_BEGIN
+  auto g = std::make_shared<CsrGraph>();
+  g->n = n;
+  g->nnz = nnz;
+  g->rowptr.assign(rowptr, rowptr + n + 1);
+  g->colidx.assign(colidx, colidx + nnz);
+  std::string err;
+  int rc = check_simple(*g, err);
+  if (rc) return fail(rc, err);
+  return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_write(const mgpm_graph *g, const char *prefix) {
+  if (!g || !prefix) return fail(MGPM_ERR_INVALID, "null argument");
+  MGPM_GUARD_BEGIN
+  std::string err;
+  int rc = write_graph(prefix, *g->g, err);
+  return rc ? fail(rc, err) : MGPM_OK;
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_info_get(const mgpm_graph *g, mgpm_graph_info *info) {
+  if (!g || !info) return fail(MGPM_ERR_INVALID, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->n = g->g->n;
+  info->nnz = g->g->nnz;
+  info->max_degree = g->g->max_degree();
+  info->meta_max_degree = g->g->meta_max_degree;
+  return MGPM_OK;
+}
+
+int mgpm_graph_csr(const mgpm_graph *g, const int64_t **rowptr,
+                   const int32_t **colidx) {
+  if (!g || !rowptr || !colidx) return fail(MGPM_ERR_INVALID, "null argument");
+  *rowptr = g->g->rowptr.data();
+  *colidx = g->g->colidx.data();
+  return MGPM_OK;
+}
+
+int mgpm_graph_free(mgpm_graph *g) {
+  delete g;
+  return MGPM_OK;
+}
+
+int mgpm_graph_generate_rmat(int32_t scale, int32_t edge_factor, double a,
+                             double b, double c, uint64_t seed,
+                             mgpm_graph **out) {
+  if (!out || scale < 1 || scale > 30 || edge_factor < 1 ||
+      (int64_t(edge_factor) << scale) > (int64_t(1) << 34) || a < 0 ||
+      b < 0 || c < 0 || a + b + c > 1.0)
+    return fail(MGPM_ERR_INVALID, "bad RMAT parameters");
+  MGPM_GUARD_BEGIN
+  auto g = std::make_shared<CsrGraph>();
+  generate_rmat(scale, edge_factor, a, b, c, seed, *g);
+  return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_generate_er(int32_t n, int64_t draws, uint64_t seed,
+                           mgpm_graph **out) {
+  if (!out || n < 1 || draws < 0 || draws > (int64_t(1) << 34))
+    return fail(MGPM_ERR_INVALID, "bad ER parameters");
+  MGPM_GUARD_BEGIN
+  auto g = std::make_shared<CsrGraph>();
+  generate_er(n, draws, seed, *g);
+  return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_orientation(const mgpm_graph *g, int32_t deg_thd,
+                           int64_t sizes[3], int64_t *dag_rowptr,
+                           int32_t *dag_colidx, int64_t *small_rowptr,
+                           int32_t *small_colidx, int64_t *large_rowptr,
+                           int32_t *large_colidx) {
+  if (!g || !sizes) return fail(MGPM_ERR_INVALID, "null argument");
+  MGPM_GUARD_BEGIN
+  const CsrGraph &G = *g->g;
+  Dag d;
+  build_dag(G, d);
+  Division dv;
+  build_division(G, d, deg_thd, dv);
+  sizes[0] = d.rowptr[G.n];
+  sizes[1] = dv.small_rowptr[G.n];
+  sizes[2] = dv.large_rowptr[G.n];
+  const size_t rp = (size_t(G.n) + 1) * sizeof(int64_t);
+  if (dag_rowptr) std::memcpy(dag_rowptr, d.rowptr.data(), rp);
+  if (dag_colidx) std::memcpy(dag_colidx, d.colidx.data(), sizes[0] * 4);
+  if (small_rowptr) std::memcpy(small_rowptr, dv.small_rowptr.data(), rp);
+  if (small_colidx)
+    std::memcpy(small_colidx, dv.small_colidx.data(), sizes[1] * 4);
+  if (large_rowptr) std::memcpy(large_rowptr, dv.large_rowptr.data(), rp);
+  if (large_colidx)
+    std::memcpy(large_colidx, dv.large_colidx.data(), sizes[2] * 4);
+  return MGPM_OK;
+  MGPM_GUARD_END
+}
+
+// ------------------------------------------------------------ devices
+
+int mgpm_device_count(int *count) {
+  if (!count) return fail(MGPM_ERR_INVALID, "null argument");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return MGPM_OK;
+}
+
+int mgpm_device_synchronize(int device) {
+  int rc = set_device(device);
+  if (rc) return rc;
+  MGPM_HIP(hipDeviceSynchronize());
+  return MGPM_OK;
+}
+
+int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
+                       mgpm_dgraph **out) {
+  if (!g || !out) return fail(MGPM_ERR_INVALID, "null argument");
+  MGPM_GUARD_BEGIN
+  int rc = set_device(device);
+  if (rc) return rc;
+  static bool inited = false;
+  if (!inited) {
+    if (tc_device_init()) return fail(MGPM_ERR_HIP, "kernel attribute setup");
+    inited = true;
+  }
+  std::unique_ptr<mgpm_dgraph> dg(new mgpm_dgraph);
+  dg->device = device;
+  dg->host = g->g;
+  MGPM_HIP(hipDeviceGetAttribute(&dg->cus, hipDeviceAttributeMultiprocessorCount,
+                                 device));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  MGPM_HIP(dg->scratch.alloc(256));
+  MGPM_HIP(hipMemsetAsync(dg->scratch.p, 0, 256, s));
+  MGPM_HIP(hipHostMalloc(reinterpret_cast<void **>(&dg->pinned), 64,
+                         hipHostMallocDefault));
+  MGPM_HIP(hipEventCreate(&dg->ev0));
+  MGPM_HIP(hipEventCreate(&dg->ev1));
+  const CsrGraph &G = *g->g;
+  rc = upload_vec(dg->csr.rowptr, G.rowptr, s);
+  if (rc) return rc;
+  rc = upload_vec(dg->csr.colidx, G.colidx, s);
+  if (rc) return rc;
+  rc = tc_make_resident(dg.get(), tc_params_from(nullptr), s);
+  if (rc) return rc;
+  *out = dg.release();
+  return MGPM_OK;
+  MGPM_GUARD_END
+}
+
+int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
+  if (!dg || !bytes) return fail(MGPM_ERR_INVALID, "null argument");
+  *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->scratch.bytes;
+  return MGPM_OK;
+}
+
+int mgpm_dgraph_free(mgpm_dgraph *dg) {
+  if (!dg) return MGPM_OK;
+  (void)hipSetDevice(dg->device);
+  (void)hipDeviceSynchronize();
+  if (dg->ev0) (void)hipEventDestroy(dg->ev0);
+  if (dg->ev1) (void)hipEventDestroy(dg->ev1);
+  if (dg->pinned) (void)hipHostFree(dg->pinned);
+  delete dg;
+  return MGPM_OK;
+}
+
+// ------------------------------------------------------------ counting
+
+int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
+                  uint64_t *count, mgpm_stats *stats) {
+  if (!dg || !count) return fail(MGPM_ERR_INVALID, "null argument");
+  MGPM_GUARD_BEGIN
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = set_device(dg->device);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  double prep = 0;
+  const TcPlanParams want = tc_params_from(tune);
+  if (!same_params(want, dg->tc.params)) {
+    const auto tp = std::chrono::steady_clock::now();
+    rc = tc_make_resident(dg, want, s);
+    if (rc) return rc;
+    prep = ms_since(tp);
+  }
+  TcLaunch l;
+  l.cus = dg->cus;
+  l.grid_cap = (tune && tune->grid_blocks > 0) ? tune->grid_blocks : 0;
+  MGPM_HIP(hipEventRecord(dg->ev0, s));
+  if (tc_device_launch(dg->tc.dev, l, s))
+    return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
+  MGPM_HIP(hipEventRecord(dg->ev1, s));
+  MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->tc.dev.result, 8,
+                          hipMemcpyDeviceToHost, s));
+  MGPM_HIP(hipStreamSynchronize(s));
+  float kms = 0;
+  MGPM_HIP(hipEventElapsedTime(&kms, dg->ev0, dg->ev1));
+  *count = dg->pinned[0];
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->kernel_ms = kms;
+    stats->prep_ms = prep;
+    stats->n = static_cast<uint64_t>(dg->host->n);
+    stats->m = static_cast<uint64_t>(dg->host->nnz / 2);
+    stats->m_plus = dg->tc.m_plus;
+    stats->s_plus = dg->tc.s_plus;
+    stats->probes = dg->tc.probes;
+    stats->bytes_alg =
+        8ull * (stats->n + 1) + 20ull * stats->m_plus + 4ull * stats->s_plus;
+    stats->path = 1;
+    stats->e2e_ms = ms_since(t0);
+  }
+  return MGPM_OK;
+  MGPM_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Internal definitions shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <memory>
+#include <string>
+
+#include "host_graph.hpp"
+#include "mgpm.h"
+#include "tc_device.hpp"
+#include "tc_plan.hpp"
+
+struct mgpm_graph {
+  std::shared_ptr<mgpm::CsrGraph> g;
+};
+
+namespace mgpm {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define MGPM_HIP(call)                                        \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::mgpm::hip_fail(e_, #call); \
+  } while (0)
+
+inline double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now() - t0)
+      .count();
+}
+
+// Device buffer owned by RAII.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t n) {
+    reset();
+    if (n == 0) n = 64;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    else p = nullptr;
+    return e;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// Triangle-count state resident on one device.
+struct TcResident {
+  TcPlanParams params;
+  uint64_t s_plus = 0, probes = 0, m_plus = 0;
+  int32_t max_out = 0;
+  DevBuf col, partners, tasks[kTcNumClasses];
+  TcDevicePlan dev;
+  size_t bytes() const;
+};
+
+// Undirected CSR resident on one device (pattern kernels).
+struct CsrResident {
+  DevBuf rowptr, colidx;
+  size_t bytes() const { return rowptr.bytes + colidx.bytes; }
+};
+
+}  // namespace mgpm
+
+struct mgpm_dgraph {
+  int device = 0;
+  int cus = 0;
+  std::shared_ptr<mgpm::CsrGraph> host;
+  mgpm::TcResident tc;
+  mgpm::CsrResident csr;
+  mgpm::DevBuf scratch;               // queues + result counters
+  unsigned long long *pinned = nullptr;  // host mirror of results
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};

@@ -1,0 +1,42 @@
+// Device-side triangle-count plan and its launcher.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+#include "tc_plan.hpp"
+
+namespace mgpm {
+
+// Capacities the kernels are compiled for (tables keep load factor <= 1/2).
+constexpr int kTcSmallMaxDeg = 128;     // 256-slot wave tables
+constexpr int kTcWaveMaxDeg = 1024;     // 2048-slot wave tables
+constexpr int kTcBlockLogpMax = 15;     // 32768-slot workgroup tables
+constexpr int kTcBlockMaxDeg = 16384;
+
+struct TcDevicePlan {
+  const int32_t *col = nullptr;       // DAG column array
+  uint32_t ncol = 0;                  // its length (m+)
+  const TcPartner *partners = nullptr;
+  const TcTask *tasks[kTcNumClasses] = {nullptr, nullptr, nullptr, nullptr};
+  int ntasks[kTcNumClasses] = {0, 0, 0, 0};
+  int class_max_tlen[kTcNumClasses] = {0, 0, 0, 0};
+  unsigned *queues = nullptr;         // kTcNumClasses counters (scratch)
+  unsigned long long *result = nullptr;
+};
+
+struct TcLaunch {
+  int cus = 256;      // compute units of the device
+  int grid_cap = 0;   // cap on persistent workgroups per kernel (0 = none)
+};
+
+// Enqueues reset + counting kernels on `stream`; the count lands in
+// *p.result.  Returns 0, or -1 if a launch failed.
+int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
+                     hipStream_t stream);
+
+// One-time per-process kernel attributes (large dynamic LDS).
+int tc_device_init();
+
+}  // namespace mgpm

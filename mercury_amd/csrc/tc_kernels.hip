@@ -1,0 +1,302 @@
+// Triangle-count kernels for gfx950.
+//
+// Reference counterpart: gpu_kernels/tc_challenge.cuh:4-303
+// (hashIndex_for_large_degree, binarySearch_for_small_degree).  Same sum
+// (|N+(a) ∩ N+(b)| over DAG edges), different machinery:
+//   * every edge hashes the larger DAG out-list and streams the smaller one
+//     (the reference does this only for its "large" edges);
+//   * tables live in LDS as open-addressing hash sets sized to the task
+//     (load factor <= 1/2), wave-private up to 2048 slots, workgroup-shared
+//     up to 32768 slots, with a global binary-search path beyond that
+//     (the reference overflows its 4096-entry LDS table, Appendix-B1 class);
+//   * partner lists are flattened across all 64 lanes (segment owners found
+//     with ballot + mbcnt) so short lists do not idle lanes;
+//   * one persistent dynamic queue per task class replaces the
+//     BLOCK/WARP_DYNAMIC_CATCH counters (tuning_schedules.cuh:3-13).
+#include <hip/hip_runtime.h>
+
+#include "tc_device.hpp"
+#include "wave.hpp"
+
+namespace mgpm {
+
+namespace {
+
+// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`,
+// 64 list items per step.  `mark` is a 64-int wave-private LDS scratch.
+// Returns this lane's hit count.
+template <class Probe>
+__device__ __forceinline__ uint32_t stream_partners(
+    const int32_t *__restrict__ col, uint32_t ncol,
+    const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
+    int first_batch, int batch_stride, int32_t *mark, Probe probe) {
+  const int lane = lane64();
+  uint32_t hits = 0;
+  for (int pb = first_batch * 64; pb < pcnt; pb += batch_stride * 64) {
+    const int pi = pb + lane;
+    uint32_t beg = 0;
+    int len = 0;
+    if (pi < pcnt) {
+      const TcPartner pr = partners[pbeg + pi];
+      beg = pr.beg;
+      len = static_cast<int>(pr.len);
+    }
+    const int incl = wave_incl_scan(len);
+    const int excl = incl - len;
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    for (int k = 0; k < total; k += 64) {
+      // Partners that start at or before item k: the last of them owns k.
+      const int started = __popcll(__ballot(excl <= k));
+      // Partners that start inside (k, k+64): one mark per start position.
+      mark[lane] = 0;
+      wave_sync();
+      if (excl > k && excl < k + 64) mark[excl - k] = 1;
+      wave_sync();
+      const unsigned long long starts = __ballot(mark[lane] != 0);
+      wave_sync();
+      const int owner = started - 1 + bits_below(starts) +
+                        static_cast<int>((starts >> lane) & 1ull);
+      const uint32_t obeg = static_cast<uint32_t>(__shfl(static_cast<int>(beg), owner, 64));
+      const int oexcl = __shfl(excl, owner, 64);
+      const int item = k + lane;
+      const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+      if (item < total && idx < ncol) hits += probe(col[idx]);
+    }
+  }
+  return hits;
+}
+
+struct HashProbe {
+  const int32_t *table;
+  int logp;
+  uint32_t mask;
+  __device__ __forceinline__ uint32_t operator()(int32_t w) const {
+    uint32_t h = slot_of(w, logp);
+    int32_t s = table[h];
+    while (s != w && s != -1) {
+      h = (h + 1) & mask;
+      s = table[h];
+    }
+    return s == w ? 1u : 0u;
+  }
+};
+
+struct SortedProbe {
+  const int32_t *list;  // sorted ascending, global memory
+  int len;
+  __device__ __forceinline__ uint32_t operator()(int32_t w) const {
+    int lo = 0, hi = len;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (list[mid] < w) lo = mid + 1;
+      else hi = mid;
+    }
+    return (lo < len && list[lo] == w) ? 1u : 0u;
+  }
+};
+
+// Builds the hash set of col[tbeg, tbeg+tlen) into table[0, 2^logp) with
+// `nthreads` cooperating threads of index `tid`.  Caller clears first.
+__device__ __forceinline__ void table_insert(int32_t *table, int logp,
+                                             const int32_t *__restrict__ col,
+                                             uint32_t tbeg, int tlen, int tid,
+                                             int nthreads) {
+  const uint32_t mask = (1u << logp) - 1u;
+  for (int i = tid; i < tlen; i += nthreads) {
+    const int32_t key = col[tbeg + i];
+    uint32_t h = slot_of(key, logp);
+    while (atomicCAS(&table[h], -1, key) != -1) h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ void flush_hits(unsigned long long acc,
+                                           unsigned long long *result) {
+  acc = wave_sum_u64(acc);
+  if (lane64() == 0 && acc) atomicAdd(result, acc);
+}
+
+// Wave-private tables of up to 2^LOGP_MAX slots (tasks with 2*d+ <= that).
+template <int LOGP_MAX, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
+    const int32_t *__restrict__ col, uint32_t ncol,
+    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
+  constexpr int PMAX = 1 << LOGP_MAX;
+  __shared__ int32_t s_table[WAVES][PMAX];
+  __shared__ int32_t s_mark[WAVES][64];
+  const int lane = lane64();
+  const int wv = threadIdx.x >> 6;
+  int32_t *table = s_table[wv];
+  unsigned long long acc = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1u);
+    t = wave_first(t);
+    if (t >= static_cast<uint32_t>(ntasks)) break;
+    const TcTask task = tasks[t];
+    if (2 * task.tlen > PMAX) {  // never planned; kept exact regardless
+      SortedProbe probe{col + task.tbeg, task.tlen};
+      acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+                             s_mark[wv], probe);
+      continue;
+    }
+    int logp = ceil_log2(4 * task.tlen);
+    logp = logp < 6 ? 6 : (logp > LOGP_MAX ? LOGP_MAX : logp);
+    const int P = 1 << logp;
+    for (int i = lane; i < P; i += 64) table[i] = -1;
+    wave_sync();
+    table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
+    wave_sync();
+    HashProbe probe{table, logp, static_cast<uint32_t>(P - 1)};
+    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+                           s_mark[wv], probe);
+    wave_sync();
+  }
+  flush_hits(acc, result);
+}
+
+// One table per workgroup (dynamic LDS of 2^logp_max slots).
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
+    const int32_t *__restrict__ col, uint32_t ncol,
+    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
+    int logp_max) {
+  extern __shared__ int32_t s_table[];
+  __shared__ int32_t s_mark[WAVES][64];
+  __shared__ uint32_t s_task;
+  const int wv = threadIdx.x >> 6;
+  unsigned long long acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_task = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t t = s_task;
+    if (t >= static_cast<uint32_t>(ntasks)) break;
+    const TcTask task = tasks[t];
+    if (2 * task.tlen > (1 << logp_max)) {  // never planned; kept exact
+      SortedProbe probe{col + task.tbeg, task.tlen};
+      acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, wv, WAVES,
+                             s_mark[wv], probe);
+      __syncthreads();
+      continue;
+    }
+    int logp = ceil_log2(2 * task.tlen);
+    logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
+    const int P = 1 << logp;
+    for (int i = threadIdx.x; i < P; i += WAVES * 64) s_table[i] = -1;
+    __syncthreads();
+    table_insert(s_table, logp, col, task.tbeg, task.tlen, threadIdx.x,
+                 WAVES * 64);
+    __syncthreads();
+    HashProbe probe{s_table, logp, static_cast<uint32_t>(P - 1)};
+    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, wv, WAVES,
+                           s_mark[wv], probe);
+    __syncthreads();
+  }
+  flush_hits(acc, result);
+}
+
+// Tables too large for LDS: binary search in the global sorted out-list.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void tc_global_search(
+    const int32_t *__restrict__ col, uint32_t ncol,
+    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
+  __shared__ int32_t s_mark[WAVES][64];
+  const int lane = lane64();
+  const int wv = threadIdx.x >> 6;
+  unsigned long long acc = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1u);
+    t = wave_first(t);
+    if (t >= static_cast<uint32_t>(ntasks)) break;
+    const TcTask task = tasks[t];
+    SortedProbe probe{col + task.tbeg, task.tlen};
+    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+                           s_mark[wv], probe);
+  }
+  flush_hits(acc, result);
+}
+
+__global__ void tc_reset(unsigned *queues, int nq,
+                         unsigned long long *result) {
+  const int i = threadIdx.x;
+  if (i < nq) queues[i] = 0;
+  if (i == 0) *result = 0;
+}
+
+constexpr int kWaves = 4;
+constexpr int kSmallLogp = 8;    // 256 slots: d+ <= 128
+constexpr int kMediumLogp = 11;  // 2048 slots: d+ <= 1024
+constexpr int kBlockWaves = 8;
+
+int grid_for(const void *kernel, int threads, size_t dyn_lds, int cus,
+             int cap) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads,
+                                                   dyn_lds) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  int g = per_cu * cus;
+  if (cap > 0 && g > cap) g = cap;
+  return g;
+}
+
+}  // namespace
+
+int tc_device_init() {
+  const hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves>),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      static_cast<int>(sizeof(int32_t) << kTcBlockLogpMax));
+  return e == hipSuccess ? 0 : -1;
+}
+
+int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
+                     hipStream_t stream) {
+  hipLaunchKernelGGL(tc_reset, dim3(1), dim3(64), 0, stream, p.queues,
+                     kTcNumClasses, p.result);
+  // Heaviest classes first so their long tasks start early.
+  if (p.ntasks[kTcBlock] > 0) {
+    int logp = 6;
+    while ((1 << logp) < 2 * p.class_max_tlen[kTcBlock]) ++logp;
+    if (logp > kTcBlockLogpMax) return -1;
+    const size_t lds = sizeof(int32_t) << logp;
+    const void *k = reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves>);
+    const int grid = grid_for(k, kBlockWaves * 64, lds, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(tc_block_tables<kBlockWaves>, dim3(grid),
+                       dim3(kBlockWaves * 64), lds, stream, p.col, p.ncol,
+                       p.tasks[kTcBlock], p.ntasks[kTcBlock], p.partners,
+                       p.queues + kTcBlock, p.result, logp);
+  }
+  if (p.ntasks[kTcMedium] > 0) {
+    const void *k = reinterpret_cast<const void *>(
+        &tc_wave_tables<kMediumLogp, kWaves>);
+    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL((tc_wave_tables<kMediumLogp, kWaves>), dim3(grid),
+                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
+                       p.tasks[kTcMedium], p.ntasks[kTcMedium], p.partners,
+                       p.queues + kTcMedium, p.result);
+  }
+  if (p.ntasks[kTcSmall] > 0) {
+    const void *k = reinterpret_cast<const void *>(
+        &tc_wave_tables<kSmallLogp, kWaves>);
+    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL((tc_wave_tables<kSmallLogp, kWaves>), dim3(grid),
+                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
+                       p.tasks[kTcSmall], p.ntasks[kTcSmall], p.partners,
+                       p.queues + kTcSmall, p.result);
+  }
+  if (p.ntasks[kTcGlobal] > 0) {
+    const void *k = reinterpret_cast<const void *>(&tc_global_search<kWaves>);
+    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(tc_global_search<kWaves>, dim3(grid),
+                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
+                       p.tasks[kTcGlobal], p.ntasks[kTcGlobal], p.partners,
+                       p.queues + kTcGlobal, p.result);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace mgpm

@@ -1,0 +1,125 @@
+// Triangle-count plan builder (host, OpenMP).
+#include "tc_plan.hpp"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "mgpm.h"
+
+namespace mgpm {
+
+int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
+  Dag dag;
+  build_dag(g, dag);
+  const int32_t n = g.n;
+  plan.n = n;
+  plan.m = g.nnz / 2;
+  plan.m_plus = dag.rowptr[n];
+  plan.max_out = dag.max_out;
+  if (plan.m_plus >= (int64_t(1) << 32)) return MGPM_ERR_CAPACITY;
+  const std::vector<int32_t> &od = dag.out_deg;
+
+  uint64_t splus = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : splus)
+  for (int32_t u = 0; u < n; ++u)
+    for (int64_t i = dag.rowptr[u]; i < dag.rowptr[u + 1]; ++i)
+      splus += static_cast<uint64_t>(od[dag.colidx[i]]);
+  plan.s_plus = splus;
+
+  // Owner rule (graph.cc:253-263 applied to every edge): T owns {T,x} when
+  // d+(T) > d+(x) or (equal and T > x); edges with an empty side carry no
+  // triangle and are dropped (graph.cc:242).
+  auto owns = [&](int32_t t, int32_t x) {
+    const int32_t a = od[t], b = od[x];
+    return a >= 1 && b >= 1 && (a > b || (a == b && t > x));
+  };
+  std::vector<int64_t> pc(n, 0), poff;
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int32_t t = 0; t < n; ++t) {
+    int64_t c = 0;
+    for (int64_t i = g.rowptr[t]; i < g.rowptr[t + 1]; ++i)
+      c += owns(t, g.colidx[i]);
+    pc[t] = c;
+  }
+  prefix_sum(pc, poff);
+  plan.partners.resize(static_cast<size_t>(poff[n]));
+  uint64_t probes = 0;
+  std::vector<int64_t> nchunks(n, 0);
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : probes)
+  for (int32_t t = 0; t < n; ++t) {
+    int64_t o = poff[t];
+    int64_t chunks = 0, acc = 0;
+    for (int64_t i = g.rowptr[t]; i < g.rowptr[t + 1]; ++i) {
+      const int32_t x = g.colidx[i];
+      if (!owns(t, x)) continue;
+      TcPartner pr;
+      pr.beg = static_cast<uint32_t>(dag.rowptr[x]);
+      pr.len = static_cast<uint32_t>(od[x]);
+      plan.partners[o++] = pr;
+      probes += pr.len;
+      if (acc == 0 || acc + pr.len > p.chunk_work) {
+        ++chunks;
+        acc = 0;
+      }
+      acc += pr.len;
+    }
+    nchunks[t] = chunks;
+  }
+  plan.probes = probes;
+
+  std::vector<int64_t> coff;
+  prefix_sum(nchunks, coff);
+  std::vector<TcTask> all(static_cast<size_t>(coff[n]));
+  std::vector<int64_t> work(all.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int32_t t = 0; t < n; ++t) {
+    if (!pc[t]) continue;
+    int64_t k = coff[t];
+    int64_t acc = 0;
+    for (int64_t i = poff[t]; i < poff[t + 1]; ++i) {
+      const uint32_t len = plan.partners[i].len;
+      if (i == poff[t] || acc + len > p.chunk_work) {
+        if (i != poff[t]) ++k;
+        all[k].tbeg = static_cast<uint32_t>(dag.rowptr[t]);
+        all[k].tlen = od[t];
+        all[k].pbeg = static_cast<uint32_t>(i);
+        all[k].pcnt = 0;
+        work[k] = 0;
+        acc = 0;
+      }
+      all[k].pcnt += 1;
+      work[k] += len;
+      acc += len;
+    }
+  }
+
+  auto cls = [&](int32_t tlen) {
+    if (tlen <= p.small_max) return kTcSmall;
+    if (tlen <= p.wave_max) return kTcMedium;
+    if (tlen <= p.block_max) return kTcBlock;
+    return kTcGlobal;
+  };
+  std::vector<int64_t> order(all.size());
+  std::iota(order.begin(), order.end(), 0);
+  // Longest tasks first (LPT) so the dynamic queue drains evenly.
+  std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    const int64_t wa = work[a] + all[a].tlen, wb = work[b] + all[b].tlen;
+    if (wa != wb) return wa > wb;
+    return a < b;
+  });
+  for (int c = 0; c < kTcNumClasses; ++c) {
+    plan.tasks[c].clear();
+    plan.class_max_tlen[c] = 0;
+  }
+  for (int64_t i : order) {
+    const int c = cls(all[i].tlen);
+    plan.tasks[c].push_back(all[i]);
+    plan.class_max_tlen[c] = std::max(plan.class_max_tlen[c], all[i].tlen);
+  }
+  plan.dag_col.swap(dag.colidx);
+  return MGPM_OK;
+}
+
+}  // namespace mgpm

@@ -1,0 +1,67 @@
+// Wave64 building blocks for gfx950 (CDNA4): every helper assumes the full
+// 64-lane wavefront is active when called.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mgpm {
+
+__device__ __forceinline__ int lane64() { return __lane_id(); }
+
+// Cross-lane handoff through LDS inside one wavefront.  The hardware runs a
+// wave's LDS instructions in order, but without a fence the compiler may
+// forward a lane's own store into its later load (or reorder them), so
+// every write-by-one-lane / read-by-another step goes through this.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of set bits of `m` strictly below the calling lane.
+__device__ __forceinline__ int bits_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(
+                                       static_cast<uint32_t>(m), 0u));
+}
+
+// Inclusive prefix sum across the wave.
+__device__ __forceinline__ int wave_incl_scan(int x) {
+  const int lane = lane64();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(
+    unsigned long long x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+__device__ __forceinline__ int wave_sum_i32(int x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
+  return x;
+}
+
+// Uniform broadcast of lane 0's value (scalar register).
+__device__ __forceinline__ uint32_t wave_first(uint32_t x) {
+  return __builtin_amdgcn_readfirstlane(x);
+}
+
+// Multiplicative (Fibonacci) hash of a vertex id into 2^logp slots.
+__device__ __forceinline__ uint32_t slot_of(int32_t key, int logp) {
+  return (static_cast<uint32_t>(key) * 0x9E3779B1u) >> (32 - logp);
+}
+
+// ceil(log2(x)) for x >= 2.
+__device__ __forceinline__ int ceil_log2(int x) { return 32 - __clz(x - 1); }
+
+}  // namespace mgpm

@@ -1,0 +1,283 @@
+"""ctypes binding of ``libmgpm.so`` (the C ABI in ``include/mgpm.h``).
+
+This is a thin host-side mirror for tests, benchmarks and Python callers;
+the counting itself always runs in the native library's HIP kernels.  If the
+library has not been built, importing raises instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libmgpm.so"
+
+TC, P1, P2, P3, P4, P5, P6, P7, P8 = range(9)
+PATTERN_NAMES = {"TC": TC, **{f"P{i}": i for i in range(1, 9)}}
+
+
+class MgpmError(RuntimeError):
+    def __init__(self, code: int, where: str, detail: str):
+        self.code = code
+        super().__init__(f"{where}: {strerror(code)} ({code}): {detail}")
+
+
+class Tune(C.Structure):
+    _fields_ = [("small_max_deg", C.c_int32), ("wave_max_deg", C.c_int32),
+                ("block_max_deg", C.c_int32), ("chunk_work", C.c_int32),
+                ("grid_blocks", C.c_int32), ("pattern_waves", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("e2e_ms", C.c_double),
+                ("prep_ms", C.c_double), ("n", C.c_uint64),
+                ("m", C.c_uint64), ("m_plus", C.c_uint64),
+                ("s_plus", C.c_uint64), ("probes", C.c_uint64),
+                ("bytes_alg", C.c_uint64), ("path", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_
+                if name != "reserved"}
+
+
+class GraphInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("max_degree", C.c_int32),
+                ("nnz", C.c_int64), ("meta_max_degree", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+def _load() -> C.CDLL:
+    if not _LIB_PATH.exists():
+        raise ImportError(
+            f"{_LIB_PATH} is not built; run `python -m mercury_amd.build` "
+            "(there is no non-native fallback)")
+    lib = C.CDLL(str(_LIB_PATH))
+    P = C.c_void_p
+    i32, i64, u64 = C.c_int32, C.c_int64, C.c_uint64
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "mgpm_strerror": (C.c_char_p, [C.c_int]),
+        "mgpm_last_error": (C.c_char_p, []),
+        "mgpm_abi_version": (C.c_int, []),
+        "mgpm_graph_load": (C.c_int, [C.c_char_p, pp]),
+        "mgpm_graph_from_csr": (C.c_int, [i32, i64, P, P, pp]),
+        "mgpm_graph_write": (C.c_int, [P, C.c_char_p]),
+        "mgpm_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
+        "mgpm_graph_csr": (C.c_int, [P, pp, pp]),
+        "mgpm_graph_free": (C.c_int, [P]),
+        "mgpm_graph_generate_rmat": (C.c_int, [i32, i32, C.c_double,
+                                               C.c_double, C.c_double, u64,
+                                               pp]),
+        "mgpm_graph_generate_er": (C.c_int, [i32, i64, u64, pp]),
+        "mgpm_graph_orientation": (C.c_int, [P, i32, P, P, P, P, P, P, P]),
+        "mgpm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+        "mgpm_device_synchronize": (C.c_int, [C.c_int]),
+        "mgpm_dgraph_upload": (C.c_int, [P, C.c_int, P, pp]),
+        "mgpm_dgraph_footprint": (C.c_int, [P, C.POINTER(u64)]),
+        "mgpm_dgraph_free": (C.c_int, [P]),
+        "mgpm_tc_count": (C.c_int, [P, C.POINTER(Tune), P, C.POINTER(u64),
+                                    C.POINTER(Stats)]),
+        "mgpm_pattern_count": (C.c_int, [P, C.c_int, C.POINTER(Tune), P,
+                                         C.POINTER(u64), C.POINTER(Stats)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+# Every symbol include/mgpm.h declares (checked by the CPU test suite).
+EXPORTED = [
+    "mgpm_strerror", "mgpm_abi_version", "mgpm_last_error",
+    "mgpm_graph_load", "mgpm_graph_from_csr", "mgpm_graph_write",
+    "mgpm_graph_info_get", "mgpm_graph_csr", "mgpm_graph_free",
+    "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
+    "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
+    "mgpm_dgraph_upload", "mgpm_dgraph_footprint", "mgpm_dgraph_free",
+    "mgpm_tc_count", "mgpm_pattern_count",
+]
+
+
+def strerror(code: int) -> str:
+    return lib.mgpm_strerror(code).decode()
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise MgpmError(code, where, lib.mgpm_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    check(lib.mgpm_device_count(C.byref(c)), "mgpm_device_count")
+    return c.value
+
+
+def make_tune(**kw) -> Tune:
+    t = Tune()
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    return t
+
+
+class Graph:
+    """Host CSR graph (``mgpm_graph``)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    # -- constructors ------------------------------------------------------
+    @classmethod
+    def load(cls, prefix: str) -> "Graph":
+        h = C.c_void_p()
+        check(lib.mgpm_graph_load(str(prefix).encode(), C.byref(h)),
+              f"mgpm_graph_load({prefix})")
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, rowptr, colidx) -> "Graph":
+        rp = np.ascontiguousarray(rowptr, dtype=np.int64)
+        ci = np.ascontiguousarray(colidx, dtype=np.int32)
+        h = C.c_void_p()
+        check(lib.mgpm_graph_from_csr(len(rp) - 1, len(ci), _ptr(rp),
+                                      _ptr(ci), C.byref(h)),
+              "mgpm_graph_from_csr")
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, a: float = 0.57,
+             b: float = 0.19, c: float = 0.19, seed: int = 1) -> "Graph":
+        h = C.c_void_p()
+        check(lib.mgpm_graph_generate_rmat(scale, edge_factor, a, b, c, seed,
+                                           C.byref(h)),
+              "mgpm_graph_generate_rmat")
+        return cls(h)
+
+    @classmethod
+    def er(cls, n: int, draws: int, seed: int) -> "Graph":
+        h = C.c_void_p()
+        check(lib.mgpm_graph_generate_er(n, draws, seed, C.byref(h)),
+              "mgpm_graph_generate_er")
+        return cls(h)
+
+    # -- accessors -----------------------------------------------------------
+    def info(self) -> GraphInfo:
+        gi = GraphInfo()
+        check(lib.mgpm_graph_info_get(self._h, C.byref(gi)), "graph_info")
+        return gi
+
+    @property
+    def n(self) -> int:
+        return self.info().n
+
+    @property
+    def nnz(self) -> int:
+        return self.info().nnz
+
+    def csr(self) -> tuple[np.ndarray, np.ndarray]:
+        """Copies of (rowptr int64[n+1], colidx int32[nnz])."""
+        gi = self.info()
+        rp, ci = C.c_void_p(), C.c_void_p()
+        check(lib.mgpm_graph_csr(self._h, C.byref(rp), C.byref(ci)),
+              "mgpm_graph_csr")
+        rowptr = np.ctypeslib.as_array(
+            C.cast(rp, C.POINTER(C.c_int64)), shape=(gi.n + 1,)).copy()
+        if gi.nnz:
+            colidx = np.ctypeslib.as_array(
+                C.cast(ci, C.POINTER(C.c_int32)), shape=(gi.nnz,)).copy()
+        else:
+            colidx = np.zeros(0, dtype=np.int32)
+        return rowptr, colidx
+
+    def write(self, prefix: str) -> None:
+        check(lib.mgpm_graph_write(self._h, str(prefix).encode()),
+              f"mgpm_graph_write({prefix})")
+
+    def orientation(self, deg_thd: int = 128) -> dict:
+        """DAG + small/large division (graph.cc:196-321 semantics)."""
+        n = self.n
+        sizes = np.zeros(3, dtype=np.int64)
+        check(lib.mgpm_graph_orientation(self._h, deg_thd, _ptr(sizes), None,
+                                         None, None, None, None, None),
+              "mgpm_graph_orientation")
+        out = {k: np.zeros(n + 1, dtype=np.int64)
+               for k in ("dag_rowptr", "small_rowptr", "large_rowptr")}
+        out["dag_colidx"] = np.zeros(max(sizes[0], 1), dtype=np.int32)
+        out["small_colidx"] = np.zeros(max(sizes[1], 1), dtype=np.int32)
+        out["large_colidx"] = np.zeros(max(sizes[2], 1), dtype=np.int32)
+        check(lib.mgpm_graph_orientation(
+            self._h, deg_thd, _ptr(sizes), _ptr(out["dag_rowptr"]),
+            _ptr(out["dag_colidx"]), _ptr(out["small_rowptr"]),
+            _ptr(out["small_colidx"]), _ptr(out["large_rowptr"]),
+            _ptr(out["large_colidx"])), "mgpm_graph_orientation")
+        for k, s in zip(("dag_colidx", "small_colidx", "large_colidx"),
+                        sizes):
+            out[k] = out[k][:s]
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib.mgpm_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceGraph:
+    """Device-resident graph (``mgpm_dgraph``) on one GPU."""
+
+    def __init__(self, graph: Graph, device: int = 0,
+                 stream: Optional[int] = None):
+        self._h = C.c_void_p()
+        check(lib.mgpm_dgraph_upload(graph._h, device,
+                                     C.c_void_p(stream or 0),
+                                     C.byref(self._h)), "mgpm_dgraph_upload")
+        self.device = device
+
+    def footprint(self) -> int:
+        b = C.c_uint64()
+        check(lib.mgpm_dgraph_footprint(self._h, C.byref(b)), "footprint")
+        return b.value
+
+    def tc_count(self, tune: Optional[Tune] = None,
+                 stream: Optional[int] = None) -> tuple[int, Stats]:
+        cnt, st = C.c_uint64(), Stats()
+        check(lib.mgpm_tc_count(self._h, C.byref(tune) if tune else None,
+                                C.c_void_p(stream or 0), C.byref(cnt),
+                                C.byref(st)), "mgpm_tc_count")
+        return cnt.value, st
+
+    def pattern_count(self, pattern: int, tune: Optional[Tune] = None,
+                      stream: Optional[int] = None) -> tuple[int, Stats]:
+        cnt, st = C.c_uint64(), Stats()
+        check(lib.mgpm_pattern_count(self._h, int(pattern),
+                                     C.byref(tune) if tune else None,
+                                     C.c_void_p(stream or 0), C.byref(cnt),
+                                     C.byref(st)), "mgpm_pattern_count")
+        return cnt.value, st
+
+    def close(self) -> None:
+        if self._h:
+            lib.mgpm_dgraph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
