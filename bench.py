@@ -1,0 +1,190 @@
+"""Benchmark of the counting path on one MI355X.
+
+Default workload: triangle count of RMAT scale 22 (edge factor 16,
+a/b/c/d = 0.57/0.19/0.19/0.05, seed 2), int32 CSR, degree-oriented, the
+largest single-GPU configuration of the headline metric.  A "step" is one
+triangle-count pass over the resident graph (all counting kernels, count
+copied back).  Prints ONE JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME]
+
+The timed region holds K steps bracketed by device synchronisation; inputs
+are resident in HBM before it starts.  `roofline.achieved` is the
+algorithmic bytes of one pass, B_alg = 8(n+1) + 20 m+ + 4 S+, divided by the
+pass's average device time from HIP events recorded on the launch stream.
+`cpu_baseline` is the CPU oracle (sorted-merge counting over the same DAG,
+OpenMP on the host cores) timed once on the same graph; its count is also
+compared with the GPU count.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+WORKLOADS = {
+    # name: (kind, scale, edge_factor, seed)
+    "tc_rmat22": ("rmat", 22, 16, 2),
+    "tc_er22": ("er", 22, 16, 3),
+    "tc_rmat20": ("rmat", 20, 16, 4),
+    "tc_rmat16": ("rmat", 16, 16, 1),
+}
+
+
+def make_graph(ma, kind, scale, ef, seed):
+    if kind == "rmat":
+        return ma.Graph.rmat(scale, ef, seed=seed)
+    n = 1 << scale
+    return ma.Graph.er(n, n * ef, seed)
+
+
+def load_traffic(workload: str):
+    """HBM bytes per pass from the committed rocprofv3 PMC summary."""
+    f = ROOT / "profiles" / f"{workload}_traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("hbm_bytes_per_pass")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(g, count_gpu, m):
+    import numpy as np  # noqa: F401
+    import oracle
+    dag = g.orientation(128)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    t0 = time.perf_counter()
+    want = oracle.tc_dag(dag["dag_rowptr"], dag["dag_colidx"], threads=threads)
+    dt = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": m / dt, "unit": "edges/s", "cores": threads, "kind": "port",
+        "sample": ("full count of the same graph: sorted-merge intersection "
+                   "over the same degree-oriented DAG (oracle/oracle.c "
+                   "orc_tc_dag), OpenMP dynamic over vertices, "
+                   f"{dt:.2f} s, {cpu_model}"),
+        "seconds": dt, "count": want, "count_matches_gpu": want == count_gpu,
+    }
+
+
+def calibrate_copy(mib: int) -> int:
+    import torch  # device memory plumbing only
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "reps": 5,
+                      "ok": bool(torch.equal(src, dst))}), flush=True)
+    return 0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="tc_rmat22", choices=WORKLOADS)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a plain device-to-device copy of MIB MiB "
+                         "(5 times) to calibrate rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE against known bytes")
+    args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the counting path here runs on a "
+              "single MI355X and has no multi-GPU mode; run with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import mercury_amd as ma
+    if ma.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    kind, scale, ef, seed = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    g = make_graph(ma, kind, scale, ef, seed)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dg = ma.DeviceGraph(g, 0)
+    upload_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        dg.tc_count()
+    ma.native.check(ma.native.lib.mgpm_device_synchronize(0), "sync")
+    kernel_ms = []
+    counts = set()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c, st = dg.tc_count()
+        kernel_ms.append(st.kernel_ms)
+        counts.add(c)
+    ma.native.check(ma.native.lib.mgpm_device_synchronize(0), "sync")
+    wall = time.perf_counter() - t0
+    count = counts.pop() if len(counts) == 1 else None
+    ms_per_step = wall * 1e3 / args.steps
+    avg_kernel_ms = sum(kernel_ms) / len(kernel_ms)
+    achieved = st.bytes_alg / (avg_kernel_ms / 1e3)
+    out = {
+        "metric": "TC edges/s",
+        "value": st.m / (ms_per_step / 1e3),
+        "unit": "edges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: triangle count, {kind.upper()} "
+                        f"scale {scale}, edge factor {ef}, seed {seed}, "
+                        "symmetrized, self-loops/duplicates removed, "
+                        "degree-oriented DAG, int32 CSR",
+            "n": st.n, "m": st.m, "m_plus": st.m_plus, "s_plus": st.s_plus,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": load_traffic(args.workload),
+            "bytes_alg": st.bytes_alg,
+            "kernel_ms_avg": avg_kernel_ms,
+        },
+        "count": count,
+        "probes": st.probes,
+        "kernel_edges_per_s": st.m / (avg_kernel_ms / 1e3),
+        "gen_s": gen_s,
+        "upload_prep_s": upload_s,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, count, st.m)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

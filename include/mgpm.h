@@ -80,10 +80,14 @@ typedef struct mgpm_tune {
   int32_t small_max_deg;
   int32_t wave_max_deg;
   int32_t block_max_deg;
-  int32_t chunk_work;     /* TC: max streamed list items per task */
+  int32_t chunk_work;     /* TC: max cost of one task (list items plus 16
+                             per partner); heavier owners are split */
   int32_t grid_blocks;    /* cap on persistent workgroups per kernel */
   int32_t pattern_waves;  /* P1..P8: waves per workgroup */
-  int32_t reserved[2];
+  int32_t group_work;     /* TC: light tasks are packed into queue entries
+                             of up to this cost */
+  int32_t tc_items;       /* TC: list items per lane per step (4 or 8;
+                             default 8) */
 } mgpm_tune;
 
 /* Per-call measurements. */

@@ -30,7 +30,8 @@ int hip_fail(hipError_t e, const char *what) {
 
 size_t TcResident::bytes() const {
   size_t b = col.bytes + partners.bytes;
-  for (int c = 0; c < kTcNumClasses; ++c) b += tasks[c].bytes;
+  for (int c = 0; c < kTcNumClasses; ++c)
+    b += tasks[c].bytes + groups[c].bytes;
   return b;
 }
 
@@ -51,6 +52,7 @@ TcPlanParams tc_params_from(const mgpm_tune *t) {
   p.block_max = std::max(
       p.wave_max, pick(t->block_max_deg, kTcBlockMaxDeg, kTcBlockMaxDeg));
   if (t->chunk_work > 0) p.chunk_work = t->chunk_work;
+  if (t->group_work > 0) p.group_work = t->group_work;
   return p;
 }
 
@@ -58,7 +60,8 @@ namespace {
 
 bool same_params(const TcPlanParams &a, const TcPlanParams &b) {
   return a.small_max == b.small_max && a.wave_max == b.wave_max &&
-         a.block_max == b.block_max && a.chunk_work == b.chunk_work;
+         a.block_max == b.block_max && a.chunk_work == b.chunk_work &&
+         a.group_work == b.group_work;
 }
 
 template <class T>
@@ -86,8 +89,11 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   if ((rc = upload_vec(r.partners, plan.partners, s))) return rc;
   for (int c = 0; c < kTcNumClasses; ++c) {
     if ((rc = upload_vec(r.tasks[c], plan.tasks[c], s))) return rc;
+    if ((rc = upload_vec(r.groups[c], plan.group_off[c], s))) return rc;
     r.dev.tasks[c] = r.tasks[c].as<TcTask>();
     r.dev.ntasks[c] = static_cast<int>(plan.tasks[c].size());
+    r.dev.groups[c] = r.groups[c].as<int32_t>();
+    r.dev.ngroups[c] = static_cast<int>(plan.group_off[c].size()) - 1;
     r.dev.class_max_tlen[c] = plan.class_max_tlen[c];
   }
   r.dev.col = r.col.as<int32_t>();
@@ -109,6 +115,8 @@ int wrap_graph(std::shared_ptr<CsrGraph> g, mgpm_graph **out) {
   return MGPM_OK;
 }
 
+}  // namespace
+
 int set_device(int device) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
@@ -118,8 +126,6 @@ int set_device(int device) {
   MGPM_HIP(hipSetDevice(device));
   return MGPM_OK;
 }
-
-}  // namespace
 
 const char *last_error_cstr() { return g_last_error.c_str(); }
 
@@ -337,7 +343,8 @@ int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
 
 int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
   if (!dg || !bytes) return fail(MGPM_ERR_INVALID, "null argument");
-  *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->scratch.bytes;
+  *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->pat.bytes() +
+           dg->scratch.bytes;
   return MGPM_OK;
 }
 
@@ -373,6 +380,8 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   TcLaunch l;
   l.cus = dg->cus;
   l.grid_cap = (tune && tune->grid_blocks > 0) ? tune->grid_blocks : 0;
+  if (tune && (tune->tc_items == 4 || tune->tc_items == 8))
+    l.items = tune->tc_items;
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (tc_device_launch(dg->tc.dev, l, s))
     return fail(MGPM_ERR_HIP, "triangle kernel launch failed");

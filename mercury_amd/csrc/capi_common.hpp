@@ -9,6 +9,7 @@
 
 #include "host_graph.hpp"
 #include "mgpm.h"
+#include "pattern_device.hpp"
 #include "tc_device.hpp"
 #include "tc_plan.hpp"
 
@@ -66,7 +67,7 @@ struct TcResident {
   TcPlanParams params;
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
   int32_t max_out = 0;
-  DevBuf col, partners, tasks[kTcNumClasses];
+  DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
   TcDevicePlan dev;
   size_t bytes() const;
 };
@@ -77,6 +78,23 @@ struct CsrResident {
   size_t bytes() const { return rowptr.bytes + colidx.bytes; }
 };
 
+// P1..P8 task plan and per-wave scratch, built on first use.
+struct PatternResident {
+  bool ready = false;
+  int grid = 0;
+  DevBuf edges, edge_groups, verts, vert_groups, slab;
+  PatternDevicePlan dev;
+  size_t bytes() const {
+    return edges.bytes + edge_groups.bytes + verts.bytes + vert_groups.bytes +
+           slab.bytes;
+  }
+};
+
+// Shared by the C-ABI units.
+TcPlanParams tc_params_from(const mgpm_tune *t);
+int set_device(int device);
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s);
+
 }  // namespace mgpm
 
 struct mgpm_dgraph {
@@ -85,6 +103,7 @@ struct mgpm_dgraph {
   std::shared_ptr<mgpm::CsrGraph> host;
   mgpm::TcResident tc;
   mgpm::CsrResident csr;
+  mgpm::PatternResident pat;
   mgpm::DevBuf scratch;               // queues + result counters
   unsigned long long *pinned = nullptr;  // host mirror of results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -1,0 +1,32 @@
+// Device-side P1..P8 plan and launcher.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace mgpm {
+
+struct PatternDevicePlan {
+  const int64_t *rowptr = nullptr;  // undirected CSR
+  const int32_t *col = nullptr;
+  const int2 *edges = nullptr;      // task edges (v0, v1), cost-ordered
+  const int32_t *edge_groups = nullptr;
+  int n_edge_groups = 0;
+  const int32_t *verts = nullptr;   // P1 vertex tasks, cost-ordered
+  const int32_t *vert_groups = nullptr;
+  int n_vert_groups = 0;
+  int32_t *slab = nullptr;          // per-wave scratch, 8 * cap ints each
+  int64_t cap = 0;                  // >= max degree
+  unsigned *queues = nullptr;       // 2 counters (edges, vertices)
+  unsigned long long *counters = nullptr;  // c1, c2
+};
+
+int pattern_waves_per_block();
+
+// Enqueues reset + the kernels of `pattern` (1..8) on `s` with `grid`
+// workgroups (the slab must hold grid * waves-per-block waves).
+int pattern_device_launch(const PatternDevicePlan &p, int pattern, int grid,
+                          hipStream_t s);
+
+}  // namespace mgpm

@@ -1,0 +1,116 @@
+// C ABI: P1..P8 pattern counts (subgraph_matching/sm_multigpu.cu:97-239 on
+// one device, without MPI).
+#include <algorithm>
+#include <cstring>
+
+#include "capi_common.hpp"
+#include "pattern_plan.hpp"
+
+namespace mgpm {
+
+namespace {
+
+// Per-wave scratch is 8 * cap ints; the persistent grid is sized to fill
+// the device but kept under this many bytes of scratch in total.
+constexpr uint64_t kSlabBudget = uint64_t(8) << 30;
+
+template <class T>
+int upload(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
+  MGPM_HIP(buf.alloc(v.size() * sizeof(T)));
+  if (!v.empty())
+    MGPM_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(T),
+                            hipMemcpyHostToDevice, s));
+  return MGPM_OK;
+}
+
+}  // namespace
+
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
+  PatternResident &r = dg->pat;
+  if (r.ready) return MGPM_OK;
+  PatternPlan plan;
+  build_pattern_plan(*dg->host, plan);
+  int rc;
+  if ((rc = upload(r.edges, plan.edges, s))) return rc;
+  if ((rc = upload(r.edge_groups, plan.edge_groups, s))) return rc;
+  if ((rc = upload(r.verts, plan.verts, s))) return rc;
+  if ((rc = upload(r.vert_groups, plan.vert_groups, s))) return rc;
+  const int64_t cap = std::max<int64_t>(plan.max_degree, 64);
+  const int wpb = pattern_waves_per_block();
+  int grid = dg->cus * 4;  // 4 workgroups of 4 waves per CU
+  const uint64_t per_block = uint64_t(wpb) * 8 * cap * sizeof(int32_t);
+  grid = static_cast<int>(
+      std::max<uint64_t>(1, std::min<uint64_t>(grid, kSlabBudget / per_block)));
+  MGPM_HIP(r.slab.alloc(per_block * grid));
+  MGPM_HIP(hipStreamSynchronize(s));
+  PatternDevicePlan &d = r.dev;
+  d.rowptr = dg->csr.rowptr.as<int64_t>();
+  d.col = dg->csr.colidx.as<int32_t>();
+  d.edges = r.edges.as<int2>();
+  d.edge_groups = r.edge_groups.as<int32_t>();
+  d.n_edge_groups = static_cast<int>(plan.edge_groups.size()) - 1;
+  d.verts = r.verts.as<int32_t>();
+  d.vert_groups = r.vert_groups.as<int32_t>();
+  d.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
+  d.slab = r.slab.as<int32_t>();
+  d.cap = cap;
+  d.queues = reinterpret_cast<unsigned *>(dg->scratch.as<char>() + 128);
+  d.counters =
+      reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 192);
+  r.grid = grid;
+  r.ready = true;
+  return MGPM_OK;
+}
+
+}  // namespace mgpm
+
+using namespace mgpm;
+
+extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
+                                  const mgpm_tune *tune, void *stream,
+                                  uint64_t *count, mgpm_stats *stats) {
+  if (!dg || !count) return fail(MGPM_ERR_INVALID, "null argument");
+  if (pattern == MGPM_TC) return mgpm_tc_count(dg, tune, stream, count, stats);
+  if (pattern < MGPM_P1 || pattern > MGPM_P8)
+    return fail(MGPM_ERR_PATTERN, "Not support.");
+  try {
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = set_device(dg->device);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    double prep = 0;
+    if (!dg->pat.ready) {
+      const auto tp = std::chrono::steady_clock::now();
+      rc = pattern_make_resident(dg, s);
+      if (rc) return rc;
+      prep = ms_since(tp);
+    }
+    MGPM_HIP(hipEventRecord(dg->ev0, s));
+    if (pattern_device_launch(dg->pat.dev, pattern, dg->pat.grid, s))
+      return fail(MGPM_ERR_HIP, "pattern kernel launch failed");
+    MGPM_HIP(hipEventRecord(dg->ev1, s));
+    MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->pat.dev.counters, 16,
+                            hipMemcpyDeviceToHost, s));
+    MGPM_HIP(hipStreamSynchronize(s));
+    float kms = 0;
+    MGPM_HIP(hipEventElapsedTime(&kms, dg->ev0, dg->ev1));
+    const uint64_t c1 = dg->pinned[0], c2 = dg->pinned[1];
+    // sm_multigpu.cu:209-232 on one rank: P4, P6 report counters[1]; the
+    // others counters[1] - counters[2] (P1: result1 - result0, the same).
+    *count = (pattern == MGPM_P4 || pattern == MGPM_P6) ? c1 : c1 - c2;
+    if (stats) {
+      std::memset(stats, 0, sizeof(*stats));
+      stats->kernel_ms = kms;
+      stats->prep_ms = prep;
+      stats->n = static_cast<uint64_t>(dg->host->n);
+      stats->m = static_cast<uint64_t>(dg->host->nnz / 2);
+      stats->path = 10 + pattern;
+      stats->e2e_ms = ms_since(t0);
+    }
+    return MGPM_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(MGPM_ERR_NOMEM, "out of host memory");
+  } catch (const std::exception &ex) {
+    return fail(MGPM_ERR_INTERNAL, ex.what());
+  }
+}

@@ -21,6 +21,8 @@ struct TcDevicePlan {
   const TcPartner *partners = nullptr;
   const TcTask *tasks[kTcNumClasses] = {nullptr, nullptr, nullptr, nullptr};
   int ntasks[kTcNumClasses] = {0, 0, 0, 0};
+  const int32_t *groups[kTcNumClasses] = {nullptr, nullptr, nullptr, nullptr};
+  int ngroups[kTcNumClasses] = {0, 0, 0, 0};
   int class_max_tlen[kTcNumClasses] = {0, 0, 0, 0};
   unsigned *queues = nullptr;         // kTcNumClasses counters (scratch)
   unsigned long long *result = nullptr;
@@ -29,6 +31,7 @@ struct TcDevicePlan {
 struct TcLaunch {
   int cus = 256;      // compute units of the device
   int grid_cap = 0;   // cap on persistent workgroups per kernel (0 = none)
+  int items = 8;      // list items per lane per step (4 or 8)
 };
 
 // Enqueues reset + counting kernels on `stream`; the count lands in

@@ -7,12 +7,17 @@
 //     (the reference does this only for its "large" edges);
 //   * tables live in LDS as open-addressing hash sets sized to the task
 //     (load factor <= 1/2), wave-private up to 2048 slots, workgroup-shared
-//     up to 32768 slots, with a global binary-search path beyond that
-//     (the reference overflows its 4096-entry LDS table, Appendix-B1 class);
-//   * partner lists are flattened across all 64 lanes (segment owners found
-//     with ballot + mbcnt) so short lists do not idle lanes;
+//     up to 32768 slots, with a global binary-search path beyond that (the
+//     reference's 4096-entry LDS table has no such path);
+//   * partner lists are flattened across all 64 lanes, 4 items per lane per
+//     step (segment owners found with ballot + mbcnt over LDS start marks),
+//     so short lists do not idle lanes and each wave keeps 4 independent
+//     gathers in flight;
 //   * one persistent dynamic queue per task class replaces the
-//     BLOCK/WARP_DYNAMIC_CATCH counters (tuning_schedules.cuh:3-13).
+//     BLOCK/WARP_DYNAMIC_CATCH counters (tuning_schedules.cuh:3-13); its
+//     entries are host-built groups of tasks of bounded cost (heavy tasks
+//     alone, light ones packed), which keeps the queue atomic off the
+//     critical path without putting several heavy tasks on one wave.
 #include <hip/hip_runtime.h>
 
 #include "tc_device.hpp"
@@ -22,14 +27,23 @@ namespace mgpm {
 
 namespace {
 
-// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`,
-// 64 list items per step.  `mark` is a 64-int wave-private LDS scratch.
-// Returns this lane's hit count.
-template <class Probe>
+constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
+
+// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`.
+// Partners are taken 64 at a time (batches first_batch, first_batch +
+// batch_stride, ...); their lists are concatenated and walked 64*ITEMS
+// items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Returns
+// this lane's hit count.
+template <int ITEMS, class Probe>
 __device__ __forceinline__ uint32_t stream_partners(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
-    int first_batch, int batch_stride, int32_t *mark, Probe probe) {
+    int first_batch, int batch_stride, int32_t *mark_lds,
+    const Probe &probe) {
+  // The start marks are written by one lane and read by another; a wave's
+  // LDS instructions execute in order, so `volatile` (no compiler
+  // reordering or store forwarding) is all the ordering needed.
+  volatile int32_t *mark = mark_lds;
   const int lane = lane64();
   uint32_t hits = 0;
   for (int pb = first_batch * 64; pb < pcnt; pb += batch_stride * 64) {
@@ -44,54 +58,86 @@ __device__ __forceinline__ uint32_t stream_partners(
     const int incl = wave_incl_scan(len);
     const int excl = incl - len;
     const int total = __builtin_amdgcn_readlane(incl, 63);
-    for (int k = 0; k < total; k += 64) {
-      // Partners that start at or before item k: the last of them owns k.
-      const int started = __popcll(__ballot(excl <= k));
-      // Partners that start inside (k, k+64): one mark per start position.
-      mark[lane] = 0;
-      wave_sync();
-      if (excl > k && excl < k + 64) mark[excl - k] = 1;
-      wave_sync();
-      const unsigned long long starts = __ballot(mark[lane] != 0);
-      wave_sync();
-      const int owner = started - 1 + bits_below(starts) +
-                        static_cast<int>((starts >> lane) & 1ull);
-      const uint32_t obeg = static_cast<uint32_t>(__shfl(static_cast<int>(beg), owner, 64));
-      const int oexcl = __shfl(excl, owner, 64);
-      const int item = k + lane;
-      const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
-      if (item < total && idx < ncol) hits += probe(col[idx]);
+    for (int k = 0; k < total; k += (64 * ITEMS)) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;
+      if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
+      unsigned long long starts[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        starts[i] = __ballot(mark[lane + 64 * i] != 0);
+      // The partner owning item g is the last one with excl <= g.
+      int base = __popcll(__ballot(excl <= k)) - 1;
+      int32_t w[ITEMS];
+      bool ok[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const int owner = base + bits_below(starts[i]) +
+                          static_cast<int>((starts[i] >> lane) & 1ull);
+        base += __popcll(starts[i]);
+        const uint32_t obeg =
+            static_cast<uint32_t>(__shfl(static_cast<int>(beg), owner, 64));
+        const int oexcl = __shfl(excl, owner, 64);
+        const int item = k + 64 * i + lane;
+        const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+        ok[i] = item < total && idx < ncol;
+        w[i] = ok[i] ? col[idx] : -2;
+      }
+      hits += probe.template lookup<ITEMS>(w, ok);
     }
   }
   return hits;
 }
 
+// Open-addressing hash set in LDS, linear probing, empty = -1.
 struct HashProbe {
   const int32_t *table;
   int logp;
   uint32_t mask;
-  __device__ __forceinline__ uint32_t operator()(int32_t w) const {
-    uint32_t h = slot_of(w, logp);
-    int32_t s = table[h];
-    while (s != w && s != -1) {
-      h = (h + 1) & mask;
-      s = table[h];
+  template <int ITEMS>
+  __device__ __forceinline__ uint32_t lookup(const int32_t (&w)[ITEMS],
+                                             const bool (&ok)[ITEMS]) const {
+    uint32_t h[ITEMS];
+    int32_t s[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      h[i] = slot_of(w[i], logp);
+      s[i] = table[h[i]];
     }
-    return s == w ? 1u : 0u;
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (!ok[i]) continue;
+      while (s[i] != w[i] && s[i] != -1) {
+        h[i] = (h[i] + 1) & mask;
+        s[i] = table[h[i]];
+      }
+      hits += s[i] == w[i] ? 1u : 0u;
+    }
+    return hits;
   }
 };
 
+// Binary search in a sorted global list (tables too large for LDS).
 struct SortedProbe {
-  const int32_t *list;  // sorted ascending, global memory
+  const int32_t *list;
   int len;
-  __device__ __forceinline__ uint32_t operator()(int32_t w) const {
-    int lo = 0, hi = len;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (list[mid] < w) lo = mid + 1;
-      else hi = mid;
+  template <int ITEMS>
+  __device__ __forceinline__ uint32_t lookup(const int32_t (&w)[ITEMS],
+                                             const bool (&ok)[ITEMS]) const {
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (!ok[i]) continue;
+      int lo = 0, hi = len;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid] < w[i]) lo = mid + 1;
+        else hi = mid;
+      }
+      hits += (lo < len && list[lo] == w[i]) ? 1u : 0u;
     }
-    return (lo < len && list[lo] == w) ? 1u : 0u;
+    return hits;
   }
 };
 
@@ -116,27 +162,30 @@ __device__ __forceinline__ void flush_hits(unsigned long long acc,
 }
 
 // Wave-private tables of up to 2^LOGP_MAX slots (tasks with 2*d+ <= that).
-template <int LOGP_MAX, int WAVES>
+template <int LOGP_MAX, int WAVES, int ITEMS>
 __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     const int32_t *__restrict__ col, uint32_t ncol,
-    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
+    int ngroups, const TcPartner *__restrict__ partners,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
   constexpr int PMAX = 1 << LOGP_MAX;
   __shared__ int32_t s_table[WAVES][PMAX];
-  __shared__ int32_t s_mark[WAVES][64];
+  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   const int lane = lane64();
   const int wv = threadIdx.x >> 6;
   int32_t *table = s_table[wv];
   unsigned long long acc = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(queue, 1u);
-    t = wave_first(t);
-    if (t >= static_cast<uint32_t>(ntasks)) break;
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(queue, 1u);
+    g = wave_first(g);
+    if (g >= static_cast<uint32_t>(ngroups)) break;
+    const int t_end = goff[g + 1];
+    for (int t = goff[g]; t < t_end; ++t) {
     const TcTask task = tasks[t];
     if (2 * task.tlen > PMAX) {  // never planned; kept exact regardless
       SortedProbe probe{col + task.tbeg, task.tlen};
-      acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
                              s_mark[wv], probe);
       continue;
     }
@@ -148,35 +197,39 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
     wave_sync();
     HashProbe probe{table, logp, static_cast<uint32_t>(P - 1)};
-    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+    acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
                            s_mark[wv], probe);
     wave_sync();
+    }
   }
   flush_hits(acc, result);
 }
 
 // One table per workgroup (dynamic LDS of 2^logp_max slots).
-template <int WAVES>
+template <int WAVES, int ITEMS>
 __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     const int32_t *__restrict__ col, uint32_t ncol,
-    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
+    int ngroups, const TcPartner *__restrict__ partners,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int logp_max) {
   extern __shared__ int32_t s_table[];
-  __shared__ int32_t s_mark[WAVES][64];
+  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   __shared__ uint32_t s_task;
   const int wv = threadIdx.x >> 6;
   unsigned long long acc = 0;
   for (;;) {
     if (threadIdx.x == 0) s_task = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t t = s_task;
-    if (t >= static_cast<uint32_t>(ntasks)) break;
+    const uint32_t g = s_task;
+    if (g >= static_cast<uint32_t>(ngroups)) break;
+    const int t_end = goff[g + 1];
+    for (int t = goff[g]; t < t_end; ++t) {
     const TcTask task = tasks[t];
     if (2 * task.tlen > (1 << logp_max)) {  // never planned; kept exact
       SortedProbe probe{col + task.tbeg, task.tlen};
-      acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, wv, WAVES,
-                             s_mark[wv], probe);
+      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
+                             WAVES, s_mark[wv], probe);
       __syncthreads();
       continue;
     }
@@ -189,32 +242,37 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
                  WAVES * 64);
     __syncthreads();
     HashProbe probe{s_table, logp, static_cast<uint32_t>(P - 1)};
-    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, wv, WAVES,
-                           s_mark[wv], probe);
+    acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
+                           WAVES, s_mark[wv], probe);
     __syncthreads();
+    }
   }
   flush_hits(acc, result);
 }
 
 // Tables too large for LDS: binary search in the global sorted out-list.
-template <int WAVES>
+template <int WAVES, int ITEMS>
 __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
     const int32_t *__restrict__ col, uint32_t ncol,
-    const TcTask *__restrict__ tasks, int ntasks, const TcPartner *__restrict__ partners,
+    const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
+    int ngroups, const TcPartner *__restrict__ partners,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
-  __shared__ int32_t s_mark[WAVES][64];
+  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   const int lane = lane64();
   const int wv = threadIdx.x >> 6;
   unsigned long long acc = 0;
   for (;;) {
-    uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(queue, 1u);
-    t = wave_first(t);
-    if (t >= static_cast<uint32_t>(ntasks)) break;
-    const TcTask task = tasks[t];
-    SortedProbe probe{col + task.tbeg, task.tlen};
-    acc += stream_partners(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
-                           s_mark[wv], probe);
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(queue, 1u);
+    g = wave_first(g);
+    if (g >= static_cast<uint32_t>(ngroups)) break;
+    const int t_end = goff[g + 1];
+    for (int t = goff[g]; t < t_end; ++t) {
+      const TcTask task = tasks[t];
+      SortedProbe probe{col + task.tbeg, task.tlen};
+      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
+                             s_mark[wv], probe);
+    }
   }
   flush_hits(acc, result);
 }
@@ -243,13 +301,63 @@ int grid_for(const void *kernel, int threads, size_t dyn_lds, int cus,
   return g;
 }
 
+template <int ITEMS>
+int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
+  // Heaviest classes first so their long tasks start early.
+  if (p.ngroups[kTcBlock] > 0) {
+    int logp = 6;
+    while ((1 << logp) < 2 * p.class_max_tlen[kTcBlock]) ++logp;
+    if (logp > kTcBlockLogpMax) return -1;
+    const size_t lds = sizeof(int32_t) << logp;
+    auto *kern = &tc_block_tables<kBlockWaves, ITEMS>;
+    const int grid = grid_for(reinterpret_cast<const void *>(kern),
+                              kBlockWaves * 64, lds, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlockWaves * 64), lds, stream,
+                       p.col, p.ncol, p.tasks[kTcBlock], p.groups[kTcBlock],
+                       p.ngroups[kTcBlock], p.partners, p.queues + kTcBlock,
+                       p.result, logp);
+  }
+  if (p.ngroups[kTcMedium] > 0) {
+    auto *kern = &tc_wave_tables<kMediumLogp, kWaves, ITEMS>;
+    const int grid = grid_for(reinterpret_cast<const void *>(kern),
+                              kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
+                       p.ncol, p.tasks[kTcMedium], p.groups[kTcMedium],
+                       p.ngroups[kTcMedium], p.partners,
+                       p.queues + kTcMedium, p.result);
+  }
+  if (p.ngroups[kTcSmall] > 0) {
+    auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS>;
+    const int grid = grid_for(reinterpret_cast<const void *>(kern),
+                              kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
+                       p.ncol, p.tasks[kTcSmall], p.groups[kTcSmall],
+                       p.ngroups[kTcSmall], p.partners, p.queues + kTcSmall,
+                       p.result);
+  }
+  if (p.ngroups[kTcGlobal] > 0) {
+    auto *kern = &tc_global_search<kWaves, ITEMS>;
+    const int grid = grid_for(reinterpret_cast<const void *>(kern),
+                              kWaves * 64, 0, l.cus, l.grid_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
+                       p.ncol, p.tasks[kTcGlobal], p.groups[kTcGlobal],
+                       p.ngroups[kTcGlobal], p.partners, p.queues + kTcGlobal,
+                       p.result);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 }  // namespace
 
 int tc_device_init() {
-  const hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves>),
-      hipFuncAttributeMaxDynamicSharedMemorySize,
-      static_cast<int>(sizeof(int32_t) << kTcBlockLogpMax));
+  const int lds = static_cast<int>(sizeof(int32_t) << kTcBlockLogpMax);
+  hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 4>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 8>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   return e == hipSuccess ? 0 : -1;
 }
 
@@ -257,46 +365,8 @@ int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
                      hipStream_t stream) {
   hipLaunchKernelGGL(tc_reset, dim3(1), dim3(64), 0, stream, p.queues,
                      kTcNumClasses, p.result);
-  // Heaviest classes first so their long tasks start early.
-  if (p.ntasks[kTcBlock] > 0) {
-    int logp = 6;
-    while ((1 << logp) < 2 * p.class_max_tlen[kTcBlock]) ++logp;
-    if (logp > kTcBlockLogpMax) return -1;
-    const size_t lds = sizeof(int32_t) << logp;
-    const void *k = reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves>);
-    const int grid = grid_for(k, kBlockWaves * 64, lds, l.cus, l.grid_cap);
-    hipLaunchKernelGGL(tc_block_tables<kBlockWaves>, dim3(grid),
-                       dim3(kBlockWaves * 64), lds, stream, p.col, p.ncol,
-                       p.tasks[kTcBlock], p.ntasks[kTcBlock], p.partners,
-                       p.queues + kTcBlock, p.result, logp);
-  }
-  if (p.ntasks[kTcMedium] > 0) {
-    const void *k = reinterpret_cast<const void *>(
-        &tc_wave_tables<kMediumLogp, kWaves>);
-    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
-    hipLaunchKernelGGL((tc_wave_tables<kMediumLogp, kWaves>), dim3(grid),
-                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
-                       p.tasks[kTcMedium], p.ntasks[kTcMedium], p.partners,
-                       p.queues + kTcMedium, p.result);
-  }
-  if (p.ntasks[kTcSmall] > 0) {
-    const void *k = reinterpret_cast<const void *>(
-        &tc_wave_tables<kSmallLogp, kWaves>);
-    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
-    hipLaunchKernelGGL((tc_wave_tables<kSmallLogp, kWaves>), dim3(grid),
-                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
-                       p.tasks[kTcSmall], p.ntasks[kTcSmall], p.partners,
-                       p.queues + kTcSmall, p.result);
-  }
-  if (p.ntasks[kTcGlobal] > 0) {
-    const void *k = reinterpret_cast<const void *>(&tc_global_search<kWaves>);
-    const int grid = grid_for(k, kWaves * 64, 0, l.cus, l.grid_cap);
-    hipLaunchKernelGGL(tc_global_search<kWaves>, dim3(grid),
-                       dim3(kWaves * 64), 0, stream, p.col, p.ncol,
-                       p.tasks[kTcGlobal], p.ntasks[kTcGlobal], p.partners,
-                       p.queues + kTcGlobal, p.result);
-  }
-  return hipGetLastError() == hipSuccess ? 0 : -1;
+  return l.items == 8 ? launch_all<8>(p, l, stream)
+                      : launch_all<4>(p, l, stream);
 }
 
 }  // namespace mgpm

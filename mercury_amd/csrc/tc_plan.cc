@@ -59,11 +59,12 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
       pr.len = static_cast<uint32_t>(od[x]);
       plan.partners[o++] = pr;
       probes += pr.len;
-      if (acc == 0 || acc + pr.len > p.chunk_work) {
+      const int64_t cost = int64_t(pr.len) + kTcPartnerCost;
+      if (acc == 0 || acc + cost > p.chunk_work) {
         ++chunks;
         acc = 0;
       }
-      acc += pr.len;
+      acc += cost;
     }
     nchunks[t] = chunks;
   }
@@ -79,19 +80,19 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
     int64_t k = coff[t];
     int64_t acc = 0;
     for (int64_t i = poff[t]; i < poff[t + 1]; ++i) {
-      const uint32_t len = plan.partners[i].len;
-      if (i == poff[t] || acc + len > p.chunk_work) {
+      const int64_t cost = int64_t(plan.partners[i].len) + kTcPartnerCost;
+      if (i == poff[t] || acc + cost > p.chunk_work) {
         if (i != poff[t]) ++k;
         all[k].tbeg = static_cast<uint32_t>(dag.rowptr[t]);
         all[k].tlen = od[t];
         all[k].pbeg = static_cast<uint32_t>(i);
         all[k].pcnt = 0;
-        work[k] = 0;
+        work[k] = kTcTableCost + od[t];
         acc = 0;
       }
       all[k].pcnt += 1;
-      work[k] += len;
-      acc += len;
+      work[k] += cost;
+      acc += cost;
     }
   }
 
@@ -105,19 +106,31 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
   std::iota(order.begin(), order.end(), 0);
   // Longest tasks first (LPT) so the dynamic queue drains evenly.
   std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-    const int64_t wa = work[a] + all[a].tlen, wb = work[b] + all[b].tlen;
-    if (wa != wb) return wa > wb;
+    if (work[a] != work[b]) return work[a] > work[b];
     return a < b;
   });
+  int64_t group_cost[kTcNumClasses] = {0, 0, 0, 0};
   for (int c = 0; c < kTcNumClasses; ++c) {
     plan.tasks[c].clear();
+    plan.group_off[c].assign(1, 0);
     plan.class_max_tlen[c] = 0;
   }
   for (int64_t i : order) {
     const int c = cls(all[i].tlen);
+    // Close the open group of class c if this task would overflow it.
+    std::vector<int32_t> &go = plan.group_off[c];
+    const int32_t open = static_cast<int32_t>(plan.tasks[c].size()) - go.back();
+    if (open > 0 && (group_cost[c] + work[i] > p.group_work || open >= 64)) {
+      go.push_back(static_cast<int32_t>(plan.tasks[c].size()));
+      group_cost[c] = 0;
+    }
     plan.tasks[c].push_back(all[i]);
+    group_cost[c] += work[i];
     plan.class_max_tlen[c] = std::max(plan.class_max_tlen[c], all[i].tlen);
   }
+  for (int c = 0; c < kTcNumClasses; ++c)
+    if (static_cast<int32_t>(plan.tasks[c].size()) != plan.group_off[c].back())
+      plan.group_off[c].push_back(static_cast<int32_t>(plan.tasks[c].size()));
   plan.dag_col.swap(dag.colidx);
   return MGPM_OK;
 }

@@ -43,11 +43,18 @@ enum TcClass : int {
   kTcNumClasses = 4
 };
 
+// Cost model used to cut and group tasks: each streamed item costs 1, each
+// partner kTcPartnerCost (its record load and share of the segment scan),
+// each task's table build kTcTableCost plus its size.
+constexpr int64_t kTcPartnerCost = 16;
+constexpr int64_t kTcTableCost = 64;
+
 struct TcPlanParams {
   int32_t small_max = 128;
   int32_t wave_max = 1024;
   int32_t block_max = 16384;
-  int64_t chunk_work = 32768;  // max partner items per task (heavy owners)
+  int64_t chunk_work = 32768;  // max cost of one task (heavy owners split)
+  int64_t group_work = 32768;  // consecutive light tasks grouped up to this
 };
 
 struct TcPlan {
@@ -59,7 +66,10 @@ struct TcPlan {
   int32_t max_out = 0;
   std::vector<int32_t> dag_col;         // DAG column array (m_plus)
   std::vector<TcPartner> partners;      // grouped by owner, task-contiguous
-  std::vector<TcTask> tasks[kTcNumClasses];  // each sorted by work, desc
+  std::vector<TcTask> tasks[kTcNumClasses];  // each sorted by cost, desc
+  // Tasks [group_off[c][g], group_off[c][g+1]) form queue entry g of class
+  // c: heavy tasks alone, light ones packed up to group_work.
+  std::vector<int32_t> group_off[kTcNumClasses];
   int32_t class_max_tlen[kTcNumClasses] = {0, 0, 0, 0};
   uint64_t bytes_alg() const {
     return 8ull * (uint64_t(n) + 1) + 20ull * uint64_t(m_plus) + 4ull * s_plus;

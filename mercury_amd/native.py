@@ -28,7 +28,7 @@ class Tune(C.Structure):
     _fields_ = [("small_max_deg", C.c_int32), ("wave_max_deg", C.c_int32),
                 ("block_max_deg", C.c_int32), ("chunk_work", C.c_int32),
                 ("grid_blocks", C.c_int32), ("pattern_waves", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("group_work", C.c_int32), ("tc_items", C.c_int32)]
 
 
 class Stats(C.Structure):

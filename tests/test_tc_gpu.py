@@ -20,6 +20,9 @@ FORCE = {
                        block_max_deg=16384),
     "global_only": dict(small_max_deg=-1, wave_max_deg=-1, block_max_deg=-1),
     "tiny_chunks": dict(chunk_work=7),
+    "items8": dict(tc_items=8),
+    "items8_global": dict(tc_items=8, small_max_deg=-1, wave_max_deg=-1,
+                          block_max_deg=-1),
 }
 
 
