@@ -93,6 +93,7 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
     r.dev.tasks[c] = r.tasks[c].as<TcTask>();
     r.dev.ntasks[c] = static_cast<int>(plan.tasks[c].size());
     r.dev.groups[c] = r.groups[c].as<int32_t>();
+    r.group_cost[c] = plan.group_cost[c];
     r.dev.ngroups[c] = static_cast<int>(plan.group_off[c].size()) - 1;
     r.dev.class_max_tlen[c] = plan.class_max_tlen[c];
   }

@@ -68,6 +68,7 @@ struct TcResident {
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
   int32_t max_out = 0;
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
+  std::vector<int64_t> group_cost[kTcNumClasses];  // host copy, batches
   TcDevicePlan dev;
   size_t bytes() const;
 };
@@ -82,7 +83,9 @@ struct CsrResident {
 struct PatternResident {
   bool ready = false;
   int grid = 0;
+  int32_t max_degree = 0;
   DevBuf edges, edge_groups, verts, vert_groups, slab;
+  std::vector<uint64_t> edge_group_cost, vert_group_cost;  // host copies
   PatternDevicePlan dev;
   size_t bytes() const {
     return edges.bytes + edge_groups.bytes + verts.bytes + vert_groups.bytes +

@@ -1,0 +1,328 @@
+// P1..P8 device functions shared by the per-graph kernels
+// (pattern_kernels.hip) and the batched multi-graph kernel
+// (batch_kernels.hip).  See pattern_kernels.hip for the design.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pattern_device.hpp"
+#include "wave.hpp"
+
+namespace mgpm {
+namespace pdev {
+
+struct PGraph {
+  const int64_t *__restrict__ rowptr;
+  const int32_t *__restrict__ col;
+  __device__ __forceinline__ const int32_t *nb(int32_t v) const {
+    return col + rowptr[v];
+  }
+  __device__ __forceinline__ int32_t deg(int32_t v) const {
+    return static_cast<int32_t>(rowptr[v + 1] - rowptr[v]);
+  }
+};
+
+// Exact membership of key in the sorted list a[0, n).
+__device__ __forceinline__ bool bs_contains(const int32_t *__restrict__ a,
+                                            int32_t n, int32_t key) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < n && a[lo] == key;
+}
+
+// out = {x in A : x < ub, (x in B) == KEEP_IN}, order preserved.  Returns
+// the count (wave-uniform).  `out` is wave-private global scratch.
+template <bool KEEP_IN>
+__device__ __forceinline__ int32_t wave_filter(const int32_t *A, int32_t nA,
+                                               const int32_t *B, int32_t nB,
+                                               int32_t ub, int32_t *out) {
+  const int lane = lane64();
+  int32_t count = 0;
+  for (int32_t c = 0; c < nA; c += 64) {
+    const int32_t i = c + lane;
+    const bool valid = i < nA;
+    const int32_t x = valid ? A[i] : 0;
+    const bool below = valid && x < ub;
+    const bool keep = below && (bs_contains(B, nB, x) == KEEP_IN);
+    const unsigned long long m = __ballot(keep);
+    if (keep) out[count + bits_below(m)] = x;
+    count += __popcll(m);
+    // A is sorted: once a valid element reaches ub, the rest do too.
+    if (__ballot(valid && !below)) break;
+  }
+  return count;
+}
+
+// D(A,B) / D<(A,B,ub) of set_difference.cuh:129-237.
+__device__ __forceinline__ int32_t wave_diff(const int32_t *A, int32_t nA,
+                                             const int32_t *B, int32_t nB,
+                                             int32_t ub, int32_t *out) {
+  return wave_filter<false>(A, nA, B, nB, ub, out);
+}
+
+// I(A,B) / I<(A,B,ub) of set_intersect.cuh:73-193: walk the shorter list,
+// search the longer one.
+__device__ __forceinline__ int32_t wave_inter(const int32_t *A, int32_t nA,
+                                              const int32_t *B, int32_t nB,
+                                              int32_t ub, int32_t *out) {
+  return nA <= nB ? wave_filter<true>(A, nA, B, nB, ub, out)
+                  : wave_filter<true>(B, nB, A, nA, ub, out);
+}
+
+constexpr int32_t kNoUb = 0x7fffffff;
+
+enum PairMode : int {
+  kNonAdj = 0,     // x !~ y
+  kNonAdjYGt = 1,  // y > x and x !~ y
+  kAdjYGt = 2,     // y > x and x ~ y
+};
+
+// Number of pairs (x in X, y in Y) satisfying MODE; per-lane partial.
+// The outer loop runs over the shorter list (uniform element, its
+// adjacency list searched by all lanes), the lanes over the longer one.
+template <int MODE>
+__device__ __forceinline__ uint64_t pair_count(const PGraph &g,
+                                               const int32_t *X, int32_t nx,
+                                               const int32_t *Y, int32_t ny) {
+  const int lane = lane64();
+  uint64_t c = 0;
+  if (nx == 0 || ny == 0) return 0;
+  const bool outer_x = nx <= ny;
+  const int32_t *O = outer_x ? X : Y;
+  const int32_t *I = outer_x ? Y : X;
+  const int32_t no = outer_x ? nx : ny, ni = outer_x ? ny : nx;
+  for (int32_t a = 0; a < no; ++a) {
+    const int32_t o = O[a];
+    const int32_t *No = g.nb(o);
+    const int32_t dO = g.deg(o);
+    for (int32_t b = lane; b < ni; b += 64) {
+      const int32_t in = I[b];
+      const int32_t x = outer_x ? o : in, y = outer_x ? in : o;
+      if (MODE != kNonAdj && !(y > x)) continue;
+      const bool adj = bs_contains(No, dO, in);
+      c += (MODE == kAdjYGt) ? adj : !adj;
+    }
+  }
+  return c;
+}
+
+// Per-wave scratch: six frontier lists and two per-index arrays.
+struct Slab {
+  int32_t *L[6];
+  uint32_t *F;    // frequencies (lane-owned: index i by lane i mod 64)
+  uint32_t *CNT;  // per-index counts written by lane 0
+  __device__ Slab(int32_t *base, int64_t cap) {
+    for (int k = 0; k < 6; ++k) L[k] = base + k * cap;
+    F = reinterpret_cast<uint32_t *>(base + 6 * cap);
+    CNT = reinterpret_cast<uint32_t *>(base + 7 * cap);
+  }
+};
+
+__device__ __forceinline__ void zero_lane_owned(uint32_t *a, int32_t n) {
+  for (int32_t i = lane64(); i < n; i += 64) a[i] = 0;
+}
+
+// Builds both sides of "L split by adjacency to v": NON = {x in L : x !~ v}
+// and ADJ = {x in L : x ~ v} (either may be null), and adds x !~ v to the
+// lane-owned F[] when F is given.  Returns the two counts.
+__device__ __forceinline__ void split_by(const PGraph &g, int32_t v,
+                                         const int32_t *L, int32_t n,
+                                         int32_t *NON, int32_t &n_non,
+                                         int32_t *ADJ, int32_t &n_adj,
+                                         uint32_t *F) {
+  const int lane = lane64();
+  const int32_t *Nv = g.nb(v);
+  const int32_t dv = g.deg(v);
+  n_non = 0;
+  n_adj = 0;
+  for (int32_t c = 0; c < n; c += 64) {
+    const int32_t i = c + lane;
+    const bool valid = i < n;
+    const int32_t x = valid ? L[i] : 0;
+    const bool adj = valid && bs_contains(Nv, dv, x);
+    const bool non = valid && !adj;
+    if (F && non) F[i] += 1;
+    const unsigned long long mn = __ballot(non), ma = __ballot(adj);
+    if (NON && non) NON[n_non + bits_below(mn)] = x;
+    if (ADJ && adj) ADJ[n_adj + bits_below(ma)] = x;
+    n_non += __popcll(mn);
+    n_adj += __popcll(ma);
+  }
+}
+
+// Number of x in L[0, upto) with (x ~ v) == ADJ, per-lane partial.
+template <bool ADJ>
+__device__ __forceinline__ uint32_t count_prefix(const PGraph &g, int32_t v,
+                                                 const int32_t *L,
+                                                 int32_t upto) {
+  const int32_t *Nv = g.nb(v);
+  const int32_t dv = g.deg(v);
+  uint32_t c = 0;
+  for (int32_t i = lane64(); i < upto; i += 64)
+    c += bs_contains(Nv, dv, L[i]) == ADJ;
+  return c;
+}
+
+// ------------------------------------------------------------ P1 vertex
+
+// P1_frequency_count (pattern_kernels.cuh:68-133) for vertex v0:
+// f[i] = #{j > i : N0[i] !~ N0[j]}, cnt[j] = #{i < j : N0[i] !~ N0[j]},
+// c1 += sum_k cnt[k] f[k].
+__device__ void p1_vertex(const PGraph &g, int32_t v0, Slab &s,
+                          uint64_t &c1) {
+  const int lane = lane64();
+  const int32_t d0 = g.deg(v0);
+  const int32_t *N0 = g.nb(v0);
+  zero_lane_owned(s.F, d0);
+  for (int32_t j = 0; j < d0; ++j) {
+    const int32_t y = N0[j];
+    const int32_t *Ny = g.nb(y);
+    const int32_t dy = g.deg(y);
+    uint32_t cnt = 0;
+    for (int32_t i = lane; i < j; i += 64) {
+      const bool non = !bs_contains(Ny, dy, N0[i]);
+      s.F[i] += non;
+      cnt += non;
+    }
+    cnt = static_cast<uint32_t>(wave_sum_i32(static_cast<int>(cnt)));
+    if (lane == 0) s.CNT[j] = cnt;
+  }
+  wave_sync();
+  for (int32_t k = lane; k < d0; k += 64)
+    c1 += static_cast<uint64_t>(s.CNT[k]) * s.F[k];
+  wave_sync();
+}
+
+// -------------------------------------------------------------- edges
+
+template <int P>
+__device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1, Slab &s,
+                          uint64_t &c1, uint64_t &c2) {
+  const int lane = lane64();
+  const int32_t *N0 = g.nb(v0), *N1 = g.nb(v1);
+  const int32_t d0 = g.deg(v0), d1 = g.deg(v1);
+  int32_t **L = s.L;
+  if constexpr (P == 1) {  // P1_count_correction :6-66
+    if (v0 == v1) return;
+    const int32_t na = wave_diff(N0, d0, N1, d1, v1, L[0]);
+    const int32_t nb = wave_inter(N0, d0, N1, d1, v1, L[1]);
+    wave_sync();
+    c2 += pair_count<kNonAdjYGt>(g, L[1], nb, L[0], na);
+  } else if constexpr (P == 2 || P == 8) {  // :135-243, :461-572
+    const int32_t n0 = P == 2 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
+                              : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
+    const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
+    zero_lane_owned(s.F, n0);
+    wave_sync();
+    for (int32_t i = 0; i < n1; ++i) {
+      const int32_t v4 = L[1][i];
+      int32_t n2, n3;
+      // L2 = D(L0, N(v4)), L3 = I(L0, N(v4)); f[j] += L0[j] !~ v4
+      split_by(g, v4, L[0], n0, L[2], n2, L[3], n3, s.F);
+      wave_sync();
+      c2 += P == 2 ? pair_count<kNonAdjYGt>(g, L[3], n3, L[2], n2)
+                   : pair_count<kAdjYGt>(g, L[3], n3, L[2], n2);
+      wave_sync();
+    }
+    // c1 += |D<(L0,N(y),y)| f[i]  (P2)   or  |I<(L0,N(y),y)| f[i]  (P8)
+    for (int32_t i = 0; i < n0; ++i) {
+      const int32_t y = L[0][i];
+      const uint32_t f = s.F[i];
+      if (f == 0) continue;
+      const uint32_t part = P == 2 ? count_prefix<false>(g, y, L[0], i)
+                                   : count_prefix<true>(g, y, L[0], i);
+      c1 += static_cast<uint64_t>(part) * f;
+    }
+  } else if constexpr (P == 3 || P == 7) {  // :245-350, :352-459
+    const int32_t n0 = P == 3 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
+                              : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
+    zero_lane_owned(s.F, n0);
+    wave_sync();
+    for (int32_t i = 0; i < n0; ++i) {
+      const int32_t y = L[0][i];
+      const int32_t *Ny = g.nb(y);
+      const int32_t dy = g.deg(y);
+      // A = D<(L0, N(y), y) (indices < i, not adjacent), B = I(L0, N(y));
+      // f[j] += 1 for j in A; cnt = |A|.
+      int32_t na = 0, nb = 0;
+      for (int32_t c = 0; c < n0; c += 64) {
+        const int32_t j = c + lane;
+        const bool valid = j < n0;
+        const int32_t x = valid ? L[0][j] : 0;
+        const bool adj = valid && bs_contains(Ny, dy, x);
+        const bool inA = valid && j < i && !adj;
+        if (inA) s.F[j] += 1;
+        const unsigned long long ma = __ballot(inA), mb = __ballot(adj);
+        if (inA) L[1][na + bits_below(ma)] = x;
+        if (adj) L[2][nb + bits_below(mb)] = x;
+        na += __popcll(ma);
+        nb += __popcll(mb);
+      }
+      if (lane == 0) s.CNT[i] = static_cast<uint32_t>(na);
+      wave_sync();
+      c2 += pair_count<kNonAdjYGt>(g, L[2], nb, L[1], na);
+      wave_sync();
+    }
+    for (int32_t k = lane; k < n0; k += 64)
+      c1 += static_cast<uint64_t>(s.CNT[k]) * s.F[k];
+  } else if constexpr (P == 5) {  // :574-734
+    const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
+    const int32_t n1 = wave_diff(N0, d0, N1, d1, kNoUb, L[1]);
+    const int32_t n2 = wave_diff(N1, d1, N0, d0, kNoUb, L[2]);
+    zero_lane_owned(s.F, n1);
+    wave_sync();
+    for (int32_t i = 0; i < n0; ++i) {
+      const int32_t v2 = L[0][i];
+      int32_t n4, n5, unused;
+      // L4 = D(L1, N(v2)) with f[j] += L1[j] !~ v2; L5 = I(L2, N(v2))
+      split_by(g, v2, L[1], n1, L[4], n4, nullptr, unused, s.F);
+      split_by(g, v2, L[2], n2, nullptr, unused, L[5], n5, nullptr);
+      wave_sync();
+      c2 += pair_count<kNonAdj>(g, L[5], n5, L[4], n4);
+      wave_sync();
+    }
+    for (int32_t i = 0; i < n1; ++i) {
+      const uint32_t f = s.F[i];
+      if (f == 0) continue;
+      const uint32_t t = count_prefix<false>(g, L[1][i], L[2], n2);
+      c1 += static_cast<uint64_t>(t) * f;
+    }
+  } else if constexpr (P == 6) {  // :736-853
+    const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
+    const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
+    wave_sync();
+    for (int32_t i = 0; i < n0; ++i) {
+      const int32_t c = L[0][i];
+      int32_t n2, n3, unused;
+      split_by(g, c, L[0], n0, L[2], n2, nullptr, unused, nullptr);
+      split_by(g, c, L[1], n1, L[3], n3, nullptr, unused, nullptr);
+      wave_sync();
+      c1 += pair_count<kNonAdj>(g, L[2], n2, L[3], n3);
+      wave_sync();
+    }
+  } else if constexpr (P == 4) {  // :855-963, v1 < v0 only
+    if (v1 >= v0) return;
+    const int32_t n0 = wave_diff(N0, d0, N1, d1, kNoUb, L[0]);
+    const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
+    wave_sync();
+    c1 += pair_count<kNonAdj>(g, L[0], n0, L[1], n1);
+  }
+  wave_sync();
+}
+
+__device__ __forceinline__ void flush2(uint64_t c1, uint64_t c2,
+                                       unsigned long long *counters) {
+  c1 = wave_sum_u64(c1);
+  c2 = wave_sum_u64(c2);
+  if (lane64() == 0) {
+    if (c1) atomicAdd(&counters[0], static_cast<unsigned long long>(c1));
+    if (c2) atomicAdd(&counters[1], static_cast<unsigned long long>(c2));
+  }
+}
+
+}  // namespace pdev
+}  // namespace mgpm

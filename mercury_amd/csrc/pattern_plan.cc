@@ -19,7 +19,8 @@ inline int bucket_of(uint64_t cost) { return 63 - __builtin_clzll(cost | 1); }
 // consecutive ones into groups of total cost <= kGroupCost.
 template <class Item, class Cost>
 void order_and_group(int64_t count, Cost cost_of, const std::vector<Item> &in,
-                     std::vector<Item> &out, std::vector<int32_t> &groups) {
+                     std::vector<Item> &out, std::vector<int32_t> &groups,
+                     std::vector<uint64_t> &group_cost) {
   std::vector<int64_t> hist(kBuckets, 0);
   std::vector<uint8_t> b(static_cast<size_t>(count));
   for (int64_t i = 0; i < count; ++i) {
@@ -39,16 +40,21 @@ void order_and_group(int64_t count, Cost cost_of, const std::vector<Item> &in,
     cost[pos] = cost_of(i);
   }
   groups.assign(1, 0);
+  group_cost.clear();
   uint64_t acc = 0;
   for (int64_t i = 0; i < count; ++i) {
     const int32_t open = static_cast<int32_t>(i) - groups.back();
     if (open > 0 && (acc + cost[i] > kGroupCost || open >= kGroupMax)) {
       groups.push_back(static_cast<int32_t>(i));
+      group_cost.push_back(acc);
       acc = 0;
     }
     acc += cost[i];
   }
-  if (groups.back() != count) groups.push_back(static_cast<int32_t>(count));
+  if (groups.back() != count) {
+    groups.push_back(static_cast<int32_t>(count));
+    group_cost.push_back(acc);
+  }
 }
 
 }  // namespace
@@ -68,12 +74,12 @@ void build_pattern_plan(const CsrGraph &g, PatternPlan &plan) {
         return uint64_t(g.degree(coo[i].v0) + 16) *
                uint64_t(g.degree(coo[i].v1) + 16);
       },
-      coo, plan.edges, plan.edge_groups);
+      coo, plan.edges, plan.edge_groups, plan.edge_group_cost);
   std::vector<int32_t> ids(static_cast<size_t>(n));
   for (int32_t v = 0; v < n; ++v) ids[v] = v;
   order_and_group(
       n, [&](int64_t v) { return uint64_t(g.degree(v) + 16) * (g.degree(v) + 16); },
-      ids, plan.verts, plan.vert_groups);
+      ids, plan.verts, plan.vert_groups, plan.vert_group_cost);
 }
 
 }  // namespace mgpm

@@ -18,8 +18,10 @@ struct PatternTaskEdge {
 struct PatternPlan {
   std::vector<PatternTaskEdge> edges;
   std::vector<int32_t> edge_groups;  // offsets, size ngroups + 1
+  std::vector<uint64_t> edge_group_cost;
   std::vector<int32_t> verts;
   std::vector<int32_t> vert_groups;
+  std::vector<uint64_t> vert_group_cost;
   int32_t max_degree = 0;
 };
 

@@ -58,6 +58,9 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
   d.counters =
       reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 192);
   r.grid = grid;
+  r.max_degree = plan.max_degree;
+  r.edge_group_cost = std::move(plan.edge_group_cost);
+  r.vert_group_cost = std::move(plan.vert_group_cost);
   r.ready = true;
   return MGPM_OK;
 }

@@ -1,0 +1,179 @@
+// Triangle-count device functions shared by the per-graph kernels
+// (tc_kernels.hip) and the batched multi-graph kernel (batch_kernels.hip).
+// See tc_kernels.hip for the design.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tc_device.hpp"
+#include "wave.hpp"
+
+namespace mgpm {
+namespace tcdev {
+
+constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
+
+// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`.
+// Partners are taken 64 at a time (batches first_batch, first_batch +
+// batch_stride, ...); their lists are concatenated and walked 64*ITEMS
+// items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Returns
+// this lane's hit count.
+template <int ITEMS, class Probe>
+__device__ __forceinline__ uint32_t stream_partners(
+    const int32_t *__restrict__ col, uint32_t ncol,
+    const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
+    int first_batch, int batch_stride, int32_t *mark_lds,
+    const Probe &probe) {
+  // The start marks are written by one lane and read by another; a wave's
+  // LDS instructions execute in order, so `volatile` (no compiler
+  // reordering or store forwarding) is all the ordering needed.
+  volatile int32_t *mark = mark_lds;
+  const int lane = lane64();
+  uint32_t hits = 0;
+  for (int pb = first_batch * 64; pb < pcnt; pb += batch_stride * 64) {
+    const int pi = pb + lane;
+    uint32_t beg = 0;
+    int len = 0;
+    if (pi < pcnt) {
+      const TcPartner pr = partners[pbeg + pi];
+      beg = pr.beg;
+      len = static_cast<int>(pr.len);
+    }
+    const int incl = wave_incl_scan(len);
+    const int excl = incl - len;
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    for (int k = 0; k < total; k += (64 * ITEMS)) {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;
+      if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
+      unsigned long long starts[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        starts[i] = __ballot(mark[lane + 64 * i] != 0);
+      // The partner owning item g is the last one with excl <= g.
+      int base = __popcll(__ballot(excl <= k)) - 1;
+      int32_t w[ITEMS];
+      bool ok[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const int owner = base + bits_below(starts[i]) +
+                          static_cast<int>((starts[i] >> lane) & 1ull);
+        base += __popcll(starts[i]);
+        const uint32_t obeg =
+            static_cast<uint32_t>(__shfl(static_cast<int>(beg), owner, 64));
+        const int oexcl = __shfl(excl, owner, 64);
+        const int item = k + 64 * i + lane;
+        const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+        ok[i] = item < total && idx < ncol;
+        w[i] = ok[i] ? col[idx] : -2;
+      }
+      hits += probe.template lookup<ITEMS>(w, ok);
+    }
+  }
+  return hits;
+}
+
+// Open-addressing hash set in LDS, linear probing, empty = -1.
+struct HashProbe {
+  const int32_t *table;
+  int logp;
+  uint32_t mask;
+  template <int ITEMS>
+  __device__ __forceinline__ uint32_t lookup(const int32_t (&w)[ITEMS],
+                                             const bool (&ok)[ITEMS]) const {
+    uint32_t h[ITEMS];
+    int32_t s[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      h[i] = slot_of(w[i], logp);
+      s[i] = table[h[i]];
+    }
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (!ok[i]) continue;
+      while (s[i] != w[i] && s[i] != -1) {
+        h[i] = (h[i] + 1) & mask;
+        s[i] = table[h[i]];
+      }
+      hits += s[i] == w[i] ? 1u : 0u;
+    }
+    return hits;
+  }
+};
+
+// Binary search in a sorted global list (tables too large for LDS).
+struct SortedProbe {
+  const int32_t *list;
+  int len;
+  template <int ITEMS>
+  __device__ __forceinline__ uint32_t lookup(const int32_t (&w)[ITEMS],
+                                             const bool (&ok)[ITEMS]) const {
+    uint32_t hits = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (!ok[i]) continue;
+      int lo = 0, hi = len;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (list[mid] < w[i]) lo = mid + 1;
+        else hi = mid;
+      }
+      hits += (lo < len && list[lo] == w[i]) ? 1u : 0u;
+    }
+    return hits;
+  }
+};
+
+// Builds the hash set of col[tbeg, tbeg+tlen) into table[0, 2^logp) with
+// `nthreads` cooperating threads of index `tid`.  Caller clears first.
+__device__ __forceinline__ void table_insert(int32_t *table, int logp,
+                                             const int32_t *__restrict__ col,
+                                             uint32_t tbeg, int tlen, int tid,
+                                             int nthreads) {
+  const uint32_t mask = (1u << logp) - 1u;
+  for (int i = tid; i < tlen; i += nthreads) {
+    const int32_t key = col[tbeg + i];
+    uint32_t h = slot_of(key, logp);
+    while (atomicCAS(&table[h], -1, key) != -1) h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ void flush_hits(unsigned long long acc,
+                                           unsigned long long *result) {
+  acc = wave_sum_u64(acc);
+  if (lane64() == 0 && acc) atomicAdd(result, acc);
+}
+
+
+// Counts one task: builds the owner's hash set in the wave-private table
+// (2^logp_max slots, all lanes of this wave) and streams its partners.  A
+// table that would not fit falls back to binary search in global memory.
+template <int ITEMS>
+__device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
+                                              uint32_t ncol,
+                                              const TcPartner *__restrict__ partners,
+                                              const TcTask &task, int32_t *table,
+                                              int logp_max, int32_t *mark) {
+  const int lane = lane64();
+  if (2 * task.tlen > (1 << logp_max)) {
+    SortedProbe probe{col + task.tbeg, task.tlen};
+    return stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0,
+                                  1, mark, probe);
+  }
+  int logp = ceil_log2(4 * task.tlen);
+  logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
+  const int P = 1 << logp;
+  for (int i = lane; i < P; i += 64) table[i] = -1;
+  wave_sync();
+  table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
+  wave_sync();
+  HashProbe probe{table, logp, static_cast<uint32_t>(P - 1)};
+  const uint32_t hits = stream_partners<ITEMS>(col, ncol, partners, task.pbeg,
+                                               task.pcnt, 0, 1, mark, probe);
+  wave_sync();
+  return hits;
+}
+
+}  // namespace tcdev
+}  // namespace mgpm

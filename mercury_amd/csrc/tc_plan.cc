@@ -113,6 +113,7 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
   for (int c = 0; c < kTcNumClasses; ++c) {
     plan.tasks[c].clear();
     plan.group_off[c].assign(1, 0);
+    plan.group_cost[c].clear();
     plan.class_max_tlen[c] = 0;
   }
   for (int64_t i : order) {
@@ -122,6 +123,7 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
     const int32_t open = static_cast<int32_t>(plan.tasks[c].size()) - go.back();
     if (open > 0 && (group_cost[c] + work[i] > p.group_work || open >= 64)) {
       go.push_back(static_cast<int32_t>(plan.tasks[c].size()));
+      plan.group_cost[c].push_back(group_cost[c]);
       group_cost[c] = 0;
     }
     plan.tasks[c].push_back(all[i]);
@@ -129,8 +131,10 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
     plan.class_max_tlen[c] = std::max(plan.class_max_tlen[c], all[i].tlen);
   }
   for (int c = 0; c < kTcNumClasses; ++c)
-    if (static_cast<int32_t>(plan.tasks[c].size()) != plan.group_off[c].back())
+    if (static_cast<int32_t>(plan.tasks[c].size()) != plan.group_off[c].back()) {
       plan.group_off[c].push_back(static_cast<int32_t>(plan.tasks[c].size()));
+      plan.group_cost[c].push_back(group_cost[c]);
+    }
   plan.dag_col.swap(dag.colidx);
   return MGPM_OK;
 }

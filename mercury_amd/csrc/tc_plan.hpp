@@ -70,6 +70,7 @@ struct TcPlan {
   // Tasks [group_off[c][g], group_off[c][g+1]) form queue entry g of class
   // c: heavy tasks alone, light ones packed up to group_work.
   std::vector<int32_t> group_off[kTcNumClasses];
+  std::vector<int64_t> group_cost[kTcNumClasses];  // cost of each group
   int32_t class_max_tlen[kTcNumClasses] = {0, 0, 0, 0};
   uint64_t bytes_alg() const {
     return 8ull * (uint64_t(n) + 1) + 20ull * uint64_t(m_plus) + 4ull * s_plus;
