@@ -97,6 +97,149 @@ def calibrate_copy(mib: int) -> int:
     return 0
 
 
+def batch_graphs(ma, count, lo, hi, seed=5):
+    """`count` RMAT graphs (a/b/c/d 0.57/0.19/0.19/0.05): scale log-uniform
+    in [lo, hi] (2^scale vertices, i.e. scale drawn uniformly), edge factor
+    uniform in {4, 8, 16, 32}; graph i uses seed*100003 + i."""
+    import random
+    rng = random.Random(seed)
+    specs = [(rng.randint(lo, hi), rng.choice((4, 8, 16, 32)))
+             for _ in range(count)]
+    return specs, [ma.Graph.rmat(s, ef, seed=seed * 100003 + i)
+                   for i, (s, ef) in enumerate(specs)]
+
+
+def bench_batch(ma, args) -> int:
+    names = [p.strip() for p in args.batch_patterns.split(",")]
+    pats = [ma.PATTERN_NAMES[p] for p in names]
+    lo, hi = args.batch_scales
+    t0 = time.perf_counter()
+    specs, graphs = batch_graphs(ma, args.batch_graphs, lo, hi)
+    dgs = [ma.DeviceGraph(g, 0) for g in graphs]
+    for dg in dgs:  # build pattern plans outside the timed region
+        if any(p != 0 for p in pats):
+            dg.pattern_count(4)
+    prep_s = time.perf_counter() - t0
+    queries = [(dg, p) for dg in dgs for p in pats]
+    sync = lambda: ma.native.check(ma.native.lib.mgpm_device_synchronize(0),
+                                   "sync")
+    # one launch (and one synchronisation) per query
+    for _ in range(args.warmup):
+        single = [dg.pattern_count(p)[0] for dg, p in queries]
+    sync()
+    t_single = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        single = [dg.pattern_count(p)[0] for dg, p in queries]
+        sync()
+        t_single.append(time.perf_counter() - t0)
+    # the same queries in one batch call, overlapped over the lanes
+    b = ma.Batch(0, lanes=args.batch_lanes, use_graph=args.batch_use_graph)
+    for _ in range(max(1, args.warmup)):
+        counts, st = b.run(queries)
+    sync()
+    t_batch, kms = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        counts, st = b.run(queries)
+        sync()
+        t_batch.append(time.perf_counter() - t0)
+        kms.append(st.kernel_ms)
+    nq = len(queries)
+    med = lambda x: sorted(x)[len(x) // 2]
+    out = {
+        "metric": "batched queries/s",
+        "value": nq / med(t_batch),
+        "unit": "queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": med(t_batch) * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"batch: {args.batch_graphs} RMAT graphs, "
+                               f"scale uniform in [{lo},{hi}], edge factor "
+                               "uniform in {4,8,16,32}, seed 5; queries "
+                               f"{'+'.join(names)} per graph",
+                   "queries": nq, "lanes": args.batch_lanes,
+                   "hip_graph": bool(args.batch_use_graph)},
+        "per_graph_launch_qps": nq / med(t_single),
+        "speedup_vs_per_graph_launch": med(t_single) / med(t_batch),
+        "counts_equal": counts == single,
+        "batch_kernel_ms": med(kms),
+        "graph_prep_s": prep_s,
+        "edges_total": sum(int(g.nnz) // 2 for g in graphs),
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if counts == single else 1
+
+
+def bench_serve(ma, args) -> int:
+    import numpy as np
+    names = [p.strip() for p in args.serve_patterns.split(",")]
+    pats = [ma.PATTERN_NAMES[p] for p in names]
+    lo, hi = args.serve_scales
+    rng = np.random.default_rng(6)
+    specs = [int(rng.integers(lo, hi + 1)) for _ in range(args.serve_graphs)]
+    graphs = [ma.Graph.rmat(s, 16, seed=6 * 100003 + i)
+              for i, s in enumerate(specs)]
+    n = args.serve_requests
+    arrivals = np.cumsum(rng.exponential(1e6 / args.serve_rate, size=n))
+    req = [(int(rng.integers(len(graphs))), pats[int(rng.integers(len(pats)))])
+           for _ in range(n)]
+
+    def run(slots, max_batch, lanes):
+        srv = ma.Server(0, slots=slots, max_batch=max_batch, lanes=lanes)
+        ids = [srv.add_graph(g) for g in graphs]
+        # warm-up: one request per pattern on the smallest graph
+        small = int(np.argmin(specs))
+        srv.replay([(ids[small], p, 0) for p in pats])
+        sched = [(ids[g], p, int(t)) for (g, p), t in zip(req, arrivals)]
+        t0 = time.perf_counter()
+        out = srv.replay(sched)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        lat = np.array([r.latency_ms for r in out])
+        ok = all(r.status == 0 for r in out)
+        return {"qps": n / wall, "wall_s": wall,
+                "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)),
+                "mean_batch": float(np.mean([r.batch_size for r in out])),
+                "launches": int(st.launches), "all_ok": ok,
+                "busy_ms": st.busy_ms}, [r.count for r in out]
+
+    served, counts = run(args.serve_slots, 1024, args.serve_lanes)
+    single, counts1 = run(1, 1, 1)
+    # each result is the same alone: spot-check distinct (graph, pattern)
+    dgs = {}
+    alone_ok = True
+    for (g, p), c in list(zip(req, counts))[:200]:
+        if g not in dgs:
+            dgs[g] = ma.DeviceGraph(graphs[g], 0)
+        alone_ok &= dgs[g].pattern_count(p)[0] == c
+    out = {
+        "metric": "served queries/s",
+        "value": served["qps"], "unit": "queries/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": served["wall_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"serve: {args.serve_graphs} resident RMAT "
+                               f"graphs (scale uniform in [{lo},{hi}], ef 16, "
+                               f"seed 6), Poisson arrivals at "
+                               f"{args.serve_rate:g} req/s, {n} requests, "
+                               f"pattern uniform in {{{','.join(names)}}}",
+                   "slots": args.serve_slots, "lanes": args.serve_lanes},
+        "p50_ms": served["p50_ms"], "p99_ms": served["p99_ms"],
+        "served": served,
+        "one_launch_per_request": single,
+        "qps_speedup": served["qps"] / single["qps"],
+        "counts_equal_modes": counts == counts1,
+        "counts_equal_alone_first200": alone_ok,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if (counts == counts1 and alone_ok) else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +247,26 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tc_rmat22", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", default="tc", choices=("tc", "batch", "serve"),
+                    help="tc: one graph's triangle count (default); batch: "
+                         "many graphs' queries in one launch vs one launch "
+                         "per query; serve: Poisson request stream through "
+                         "the continuous-batching server vs one launch per "
+                         "request")
+    ap.add_argument("--serve-graphs", type=int, default=64)
+    ap.add_argument("--serve-scales", type=int, nargs=2, default=(8, 12),
+                    metavar=("LO", "HI"))
+    ap.add_argument("--serve-patterns", default="TC,P1,P2,P3,P4,P5,P6,P7,P8")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-lanes", type=int, default=2)
+    ap.add_argument("--batch-lanes", type=int, default=4)
+    ap.add_argument("--batch-use-graph", type=int, default=0)
+    ap.add_argument("--batch-graphs", type=int, default=512)
+    ap.add_argument("--batch-scales", type=int, nargs=2, default=(10, 13),
+                    metavar=("LO", "HI"))
+    ap.add_argument("--batch-patterns", default="TC,P2,P4")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB "
                          "(5 times) to calibrate rocprofv3 FETCH_SIZE / "
@@ -121,6 +284,10 @@ def main() -> int:
     if ma.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.mode == "batch":
+        return bench_batch(ma, args)
+    if args.mode == "serve":
+        return bench_serve(ma, args)
     kind, scale, ef, seed = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     g = make_graph(ma, kind, scale, ef, seed)

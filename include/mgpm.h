@@ -183,6 +183,96 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
 int mgpm_pattern_count(mgpm_dgraph *dg, int pattern, const mgpm_tune *tune,
                        void *stream, uint64_t *count, mgpm_stats *stats);
 
+/* ---- batches ------------------------------------------------------------
+ * No counterpart in the reference (one graph, one pattern per process).
+ * Many (graph, pattern) queries run in one call with no host round trip per
+ * query: one fill kernel zeroes every query's queues and counters, the
+ * queries are spread over `lanes` streams (longest first, each to the
+ * least-loaded lane) where their counting kernels -- the same kernels as
+ * mgpm_tc_count / mgpm_pattern_count, with each launch capped at the
+ * workgroups its queue can feed -- overlap on the device, and one copy
+ * returns all counters.  With use_graph the fill, fork, kernels and join
+ * are recorded once into a HIP graph and replayed while the query set
+ * (graph handles, patterns, plan generations) is unchanged.  Every query
+ * uses its graph's resident plan (mgpm_autotune's choice if it ran).  All
+ * graphs must live on the batch's device. */
+typedef struct mgpm_query {
+  mgpm_dgraph *graph; /* in */
+  int32_t pattern;    /* in: MGPM_TC or MGPM_P1..MGPM_P8 */
+  int32_t status;     /* out: MGPM_OK or a negative code */
+  uint64_t count;     /* out */
+} mgpm_query;
+
+typedef struct mgpm_batch_opts {
+  int32_t lanes;      /* concurrent streams (0 = 4, at most 16) */
+  int32_t use_graph;  /* 1 = record and replay a HIP graph */
+  int32_t reserved[6];
+} mgpm_batch_opts;
+
+/* opts may be NULL (defaults). */
+int mgpm_batch_create(int device, const mgpm_batch_opts *opts,
+                      mgpm_batch **out);
+/* Runs the queries; per-query results in queries[i].count / .status.
+ * stats->kernel_ms is the device time of the whole batch (hipEvents on
+ * `stream` around fill, kernels, join and copy). */
+int mgpm_batch_run(mgpm_batch *b, mgpm_query *queries, int nq, void *stream,
+                   mgpm_stats *stats);
+/* Device bytes of the batch's own workspace (counters, queues, scratch). */
+int mgpm_batch_footprint(const mgpm_batch *b, uint64_t *bytes);
+int mgpm_batch_free(mgpm_batch *b);
+
+/* ---- continuous-batching server -------------------------------------------
+ * No counterpart in the reference.  Graphs are added once and stay resident
+ * in HBM; requests (graph id, pattern) are submitted from any thread and
+ * served by a scheduler thread that keeps `slots` batches in flight, each
+ * with its own streams and workspace: whenever a slot frees, every pending
+ * request (up to max_batch, FIFO) is admitted into one batch (the
+ * mgpm_batch_run machinery over `lanes` streams), so requests that arrive
+ * while others run join the next batch and batches on different slots
+ * overlap on the device.  Admission is by HBM footprint: a graph whose
+ * resident footprint would exceed the budget is refused (MGPM_ERR_CAPACITY)
+ * and slot workspaces are charged against the same budget.  slots = 1,
+ * lanes = 1, max_batch = 1 gives one launch per request in arrival order. */
+typedef struct mgpm_server_config {
+  int32_t device;
+  int32_t slots;       /* batches in flight (0 = 3) */
+  int32_t max_batch;   /* requests per batch (0 = 1024) */
+  int32_t lanes;       /* streams per slot (0 = 2) */
+  uint64_t hbm_budget; /* bytes for graphs + workspaces (0 = 90% of free) */
+  uint64_t reserved[2];
+} mgpm_server_config;
+
+typedef struct mgpm_request {
+  int32_t graph;       /* in: id from mgpm_server_add_graph */
+  int32_t pattern;     /* in: MGPM_TC or MGPM_P1..MGPM_P8 */
+  uint64_t arrival_us; /* in (replay only): offset from the replay start */
+  uint64_t count;      /* out */
+  int32_t status;      /* out */
+  int32_t batch_size;  /* out: requests served by the same launch */
+  double latency_ms;   /* out: completion - arrival */
+} mgpm_request;
+
+typedef struct mgpm_server_stats {
+  uint64_t submitted, completed, failed, launches;
+  uint64_t resident_bytes, workspace_bytes, budget_bytes;
+  double busy_ms; /* wall time with at least one launch in flight */
+} mgpm_server_stats;
+
+int mgpm_server_create(const mgpm_server_config *cfg, mgpm_server **out);
+int mgpm_server_add_graph(mgpm_server *s, const mgpm_graph *g,
+                          int32_t *graph_id);
+int mgpm_server_submit(mgpm_server *s, int32_t graph_id, int32_t pattern,
+                       uint64_t *ticket);
+/* Blocks until the request finishes (timeout_ms < 0: no limit); fills
+ * count/status/batch_size/latency_ms of *out (graph/pattern too). */
+int mgpm_server_wait(mgpm_server *s, uint64_t ticket, int32_t timeout_ms,
+                     mgpm_request *out);
+/* Submits reqs[i] at reqs[i].arrival_us after the call starts (open-loop,
+ * from the calling thread), then waits for all and fills the outputs. */
+int mgpm_server_replay(mgpm_server *s, mgpm_request *reqs, int n);
+int mgpm_server_stats_get(mgpm_server *s, mgpm_server_stats *st);
+int mgpm_server_destroy(mgpm_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,8 @@ pattern counts) behind the C ABI of ``include/mgpm.h``.
 The native library is loaded on first use of an exported name, so that
 ``python -m mercury_amd.build`` works before anything is built.
 """
-__all__ = ["Graph", "DeviceGraph", "Tune", "Stats", "MgpmError", "make_tune",
+__all__ = ["Graph", "DeviceGraph", "Batch", "Server", "Tune", "Stats", "MgpmError",
+           "make_tune",
            "device_count", "PATTERN_NAMES", "TC", "P1", "P2", "P3", "P4",
            "P5", "P6", "P7", "P8"]
 

@@ -37,7 +37,7 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall",
 # Translation units of the library (CLI mains are separate).
 LIB_SOURCES = [
     "host_graph.cc", "tc_plan.cc", "pattern_plan.cc", "capi.cc",
-    "patterns.cc", "tc_kernels.hip", "pattern_kernels.hip",
+    "patterns.cc", "batch.cc", "server.cc", "tc_kernels.hip", "pattern_kernels.hip",
 ]
 CLI_SOURCES = {"tc_challenge": "cli_tc_challenge.cc", "sm": "cli_sm.cc"}
 
@@ -76,7 +76,7 @@ def build(verbose: bool = True) -> Path:
                           if (CSRC / s).exists()]
     with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         objs = dict(zip(srcs, ex.map(_compile, srcs)))
-    link = ["-L" + str(ROCM / "lib"), "-lamdhip64", "-fopenmp",
+    link = ["-L" + str(ROCM / "lib"), "-lamdhip64", "-fopenmp", "-pthread",
             f"-Wl,-rpath,{ROCM / 'lib'}"]
     tmp = LIB.with_suffix(".so.tmp")
     _run(["g++", "-shared", "-o", str(tmp),

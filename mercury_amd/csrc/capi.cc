@@ -73,6 +73,8 @@ int upload_vec(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
   return MGPM_OK;
 }
 
+}  // namespace
+
 // (Re)builds the triangle plan on the host and uploads it.
 int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s) {
@@ -81,6 +83,7 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   if (rc) return fail(rc, "triangle plan: DAG has 2^32 or more arcs");
   TcResident &r = dg->tc;
   r.params = params;
+  r.generation++;
   r.s_plus = plan.s_plus;
   r.probes = plan.probes;
   r.m_plus = static_cast<uint64_t>(plan.m_plus);
@@ -93,7 +96,6 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
     r.dev.tasks[c] = r.tasks[c].as<TcTask>();
     r.dev.ntasks[c] = static_cast<int>(plan.tasks[c].size());
     r.dev.groups[c] = r.groups[c].as<int32_t>();
-    r.group_cost[c] = plan.group_cost[c];
     r.dev.ngroups[c] = static_cast<int>(plan.group_off[c].size()) - 1;
     r.dev.class_max_tlen[c] = plan.class_max_tlen[c];
   }
@@ -107,6 +109,8 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   MGPM_HIP(hipStreamSynchronize(s));
   return MGPM_OK;
 }
+
+namespace {
 
 int wrap_graph(std::shared_ptr<CsrGraph> g, mgpm_graph **out) {
   mgpm_graph *h = new (std::nothrow) mgpm_graph;
@@ -371,7 +375,9 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   double prep = 0;
-  const TcPlanParams want = tc_params_from(tune);
+  // NULL keeps the resident plan (the defaults, or what mgpm_autotune
+  // chose); an explicit tune rebuilds it when its plan knobs differ.
+  const TcPlanParams want = tune ? tc_params_from(tune) : dg->tc.params;
   if (!same_params(want, dg->tc.params)) {
     const auto tp = std::chrono::steady_clock::now();
     rc = tc_make_resident(dg, want, s);
@@ -380,9 +386,11 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   }
   TcLaunch l;
   l.cus = dg->cus;
-  l.grid_cap = (tune && tune->grid_blocks > 0) ? tune->grid_blocks : 0;
-  if (tune && (tune->tc_items == 4 || tune->tc_items == 8))
-    l.items = tune->tc_items;
+  l.grid_cap = (tune && tune->grid_blocks > 0) ? tune->grid_blocks
+                                               : dg->tc.grid_cap;
+  l.items = (tune && (tune->tc_items == 4 || tune->tc_items == 8))
+                ? tune->tc_items
+                : dg->tc.items;
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (tc_device_launch(dg->tc.dev, l, s))
     return fail(MGPM_ERR_HIP, "triangle kernel launch failed");

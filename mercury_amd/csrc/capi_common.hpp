@@ -6,6 +6,7 @@
 #include <chrono>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "host_graph.hpp"
 #include "mgpm.h"
@@ -65,10 +66,12 @@ struct DevBuf {
 // Triangle-count state resident on one device.
 struct TcResident {
   TcPlanParams params;
+  uint64_t generation = 0;  // bumped whenever the plan or launch changes
+  int items = 8;            // list items per lane per step
+  int grid_cap = 0;         // cap on persistent workgroups (0 = none)
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
   int32_t max_out = 0;
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
-  std::vector<int64_t> group_cost[kTcNumClasses];  // host copy, batches
   TcDevicePlan dev;
   size_t bytes() const;
 };
@@ -79,7 +82,8 @@ struct CsrResident {
   size_t bytes() const { return rowptr.bytes + colidx.bytes; }
 };
 
-// P1..P8 task plan and per-wave scratch, built on first use.
+// P1..P8 task plan, built on first use, and the per-wave scratch of the
+// single-query path, allocated on its first call.
 struct PatternResident {
   bool ready = false;
   int grid = 0;
@@ -96,7 +100,10 @@ struct PatternResident {
 // Shared by the C-ABI units.
 TcPlanParams tc_params_from(const mgpm_tune *t);
 int set_device(int device);
+int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
+                     hipStream_t s);
 int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s);
+int pattern_make_slab(mgpm_dgraph *dg);
 
 }  // namespace mgpm
 

@@ -24,9 +24,10 @@ struct PatternDevicePlan {
 
 int pattern_waves_per_block();
 
-// Enqueues reset + the kernels of `pattern` (1..8) on `s` with `grid`
-// workgroups (the slab must hold grid * waves-per-block waves).
+// Enqueues the kernels of `pattern` (1..8) on `s` with at most `grid`
+// workgroups (the slab must hold grid * waves-per-block waves), preceded by
+// a reset of queues and counters unless the caller has zeroed them.
 int pattern_device_launch(const PatternDevicePlan &p, int pattern, int grid,
-                          hipStream_t s);
+                          hipStream_t s, bool reset = true);
 
 }  // namespace mgpm

@@ -21,6 +21,8 @@
 // counters[2] (c2); sm_multigpu.cu:209-232 combines them on the host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pattern_device_fns.hpp"
 
 namespace mgpm {
@@ -89,20 +91,28 @@ void launch_edges(const PatternDevicePlan &p, int grid, hipStream_t s) {
                      p.cap, p.queues, p.counters);
 }
 
+// Workgroups that can take a queue entry: one entry per wave.
+int blocks_for(int groups) { return (groups + kPWaves - 1) / kPWaves; }
+
 }  // namespace
 
 int pattern_waves_per_block() { return kPWaves; }
 
 int pattern_device_launch(const PatternDevicePlan &p, int pattern, int grid,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(pattern_reset, dim3(1), dim3(64), 0, s, p.queues,
-                     p.counters);
-  if (pattern == 1) {
+                          hipStream_t s, bool reset) {
+  if (pattern < 1 || pattern > 8 || grid < 1) return -1;
+  if (reset)
+    hipLaunchKernelGGL(pattern_reset, dim3(1), dim3(64), 0, s, p.queues,
+                       p.counters);
+  if (pattern == 1 && p.n_vert_groups > 0) {
     PGraph g{p.rowptr, p.col};
-    hipLaunchKernelGGL(pattern_p1_vertices, dim3(grid), dim3(kPWaves * 64), 0,
+    const int vgrid = std::min(grid, blocks_for(p.n_vert_groups));
+    hipLaunchKernelGGL(pattern_p1_vertices, dim3(vgrid), dim3(kPWaves * 64), 0,
                        s, g, p.verts, p.vert_groups, p.n_vert_groups, p.slab,
                        p.cap, p.queues + 1, p.counters);
   }
+  if (p.n_edge_groups <= 0) return hipGetLastError() == hipSuccess ? 0 : -1;
+  grid = std::min(grid, blocks_for(p.n_edge_groups));
   switch (pattern) {
     case 1: launch_edges<1>(p, grid, s); break;
     case 2: launch_edges<2>(p, grid, s); break;

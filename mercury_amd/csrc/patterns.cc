@@ -36,12 +36,6 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
   if ((rc = upload(r.verts, plan.verts, s))) return rc;
   if ((rc = upload(r.vert_groups, plan.vert_groups, s))) return rc;
   const int64_t cap = std::max<int64_t>(plan.max_degree, 64);
-  const int wpb = pattern_waves_per_block();
-  int grid = dg->cus * 4;  // 4 workgroups of 4 waves per CU
-  const uint64_t per_block = uint64_t(wpb) * 8 * cap * sizeof(int32_t);
-  grid = static_cast<int>(
-      std::max<uint64_t>(1, std::min<uint64_t>(grid, kSlabBudget / per_block)));
-  MGPM_HIP(r.slab.alloc(per_block * grid));
   MGPM_HIP(hipStreamSynchronize(s));
   PatternDevicePlan &d = r.dev;
   d.rowptr = dg->csr.rowptr.as<int64_t>();
@@ -52,16 +46,30 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
   d.verts = r.verts.as<int32_t>();
   d.vert_groups = r.vert_groups.as<int32_t>();
   d.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
-  d.slab = r.slab.as<int32_t>();
+  d.slab = nullptr;
   d.cap = cap;
   d.queues = reinterpret_cast<unsigned *>(dg->scratch.as<char>() + 128);
   d.counters =
       reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 192);
-  r.grid = grid;
+  r.grid = 0;
   r.max_degree = plan.max_degree;
   r.edge_group_cost = std::move(plan.edge_group_cost);
   r.vert_group_cost = std::move(plan.vert_group_cost);
   r.ready = true;
+  return MGPM_OK;
+}
+
+int pattern_make_slab(mgpm_dgraph *dg) {
+  PatternResident &r = dg->pat;
+  if (r.slab.p) return MGPM_OK;
+  const int wpb = pattern_waves_per_block();
+  int grid = dg->cus * 4;  // 4 workgroups of 4 waves per CU
+  const uint64_t per_block = uint64_t(wpb) * 8 * r.dev.cap * sizeof(int32_t);
+  grid = static_cast<int>(
+      std::max<uint64_t>(1, std::min<uint64_t>(grid, kSlabBudget / per_block)));
+  MGPM_HIP(r.slab.alloc(per_block * grid));
+  r.dev.slab = r.slab.as<int32_t>();
+  r.grid = grid;
   return MGPM_OK;
 }
 
@@ -82,10 +90,10 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     double prep = 0;
-    if (!dg->pat.ready) {
+    if (!dg->pat.ready || !dg->pat.slab.p) {
       const auto tp = std::chrono::steady_clock::now();
-      rc = pattern_make_resident(dg, s);
-      if (rc) return rc;
+      if ((rc = pattern_make_resident(dg, s))) return rc;
+      if ((rc = pattern_make_slab(dg))) return rc;
       prep = ms_since(tp);
     }
     MGPM_HIP(hipEventRecord(dg->ev0, s));

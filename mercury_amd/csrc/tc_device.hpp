@@ -32,12 +32,17 @@ struct TcLaunch {
   int cus = 256;      // compute units of the device
   int grid_cap = 0;   // cap on persistent workgroups per kernel (0 = none)
   int items = 8;      // list items per lane per step (4 or 8)
+  bool reset = true;  // zero queues and result first (else the caller has)
 };
 
 // Enqueues reset + counting kernels on `stream`; the count lands in
 // *p.result.  Returns 0, or -1 if a launch failed.
 int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
                      hipStream_t stream);
+
+// Enqueues a fill of `words` 32-bit zeros at `p` (a kernel, so it may be
+// recorded into a HIP graph).  Returns 0, or -1 if the launch failed.
+int device_zero(void *p, size_t words, hipStream_t stream);
 
 // One-time per-process kernel attributes (large dynamic LDS).
 int tc_device_init();

@@ -12,6 +12,9 @@ namespace mgpm {
 namespace tcdev {
 
 constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
+// Partner batches whose lists average at least this many items use the
+// per-lane owner cursor instead of start marks.
+constexpr int kCursorMinAvgLen = 32;
 
 // Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`.
 // Partners are taken 64 at a time (batches first_batch, first_batch +
@@ -42,6 +45,39 @@ __device__ __forceinline__ uint32_t stream_partners(
     const int incl = wave_incl_scan(len);
     const int excl = incl - len;
     const int total = __builtin_amdgcn_readlane(incl, 63);
+    if (total >= kCursorMinAvgLen * 64) {
+      // Long lists on average: each lane walks its own owner cursor over
+      // the batch's prefix sums (kept in LDS: a cursor advances in
+      // divergent code, where cross-lane shuffles would read inactive
+      // lanes).  Items lane + 64 i advance the cursor by at most a few
+      // partners per step.
+      mark[lane] = incl;
+      mark[64 + lane] = static_cast<int32_t>(beg);
+      int o = 0, oexcl = 0, onext = mark[0];
+      uint32_t obeg = static_cast<uint32_t>(mark[64]);
+      for (int k = 0; k < total; k += 64 * ITEMS) {
+        int32_t w[ITEMS];
+        bool ok[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+          const int item = k + 64 * i + lane;
+          ok[i] = item < total;
+          if (ok[i]) {
+            while (item >= onext) {
+              ++o;
+              oexcl = onext;
+              onext = mark[o];
+              obeg = static_cast<uint32_t>(mark[64 + o]);
+            }
+          }
+          const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+          ok[i] = ok[i] && idx < ncol;
+          w[i] = ok[i] ? col[idx] : -2;
+        }
+        hits += probe.template lookup<ITEMS>(w, ok);
+      }
+      continue;
+    }
     for (int k = 0; k < total; k += (64 * ITEMS)) {
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;

@@ -20,6 +20,8 @@
 //     critical path without putting several heavy tasks on one wave.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "tc_device_fns.hpp"
 
 namespace mgpm {
@@ -138,8 +140,10 @@ constexpr int kSmallLogp = 8;    // 256 slots: d+ <= 128
 constexpr int kMediumLogp = 11;  // 2048 slots: d+ <= 1024
 constexpr int kBlockWaves = 8;
 
+// Persistent grid: what fits the device at once, capped by `cap` (if set)
+// and by the workgroups that can take a queue entry at all (`units`).
 int grid_for(const void *kernel, int threads, size_t dyn_lds, int cus,
-             int cap) {
+             int cap, int units) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads,
                                                    dyn_lds) != hipSuccess ||
@@ -147,8 +151,11 @@ int grid_for(const void *kernel, int threads, size_t dyn_lds, int cus,
     per_cu = 1;
   int g = per_cu * cus;
   if (cap > 0 && g > cap) g = cap;
+  if (units > 0 && g > units) g = units;
   return g;
 }
+
+int waves_to_blocks(int groups, int waves) { return (groups + waves - 1) / waves; }
 
 template <int ITEMS>
 int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
@@ -160,7 +167,8 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
     const size_t lds = sizeof(int32_t) << logp;
     auto *kern = &tc_block_tables<kBlockWaves, ITEMS>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
-                              kBlockWaves * 64, lds, l.cus, l.grid_cap);
+                              kBlockWaves * 64, lds, l.cus, l.grid_cap,
+                              p.ngroups[kTcBlock]);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlockWaves * 64), lds, stream,
                        p.col, p.ncol, p.tasks[kTcBlock], p.groups[kTcBlock],
                        p.ngroups[kTcBlock], p.partners, p.queues + kTcBlock,
@@ -169,7 +177,8 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
   if (p.ngroups[kTcMedium] > 0) {
     auto *kern = &tc_wave_tables<kMediumLogp, kWaves, ITEMS>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
-                              kWaves * 64, 0, l.cus, l.grid_cap);
+                              kWaves * 64, 0, l.cus, l.grid_cap,
+                              waves_to_blocks(p.ngroups[kTcMedium], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcMedium], p.groups[kTcMedium],
                        p.ngroups[kTcMedium], p.partners,
@@ -178,7 +187,8 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
   if (p.ngroups[kTcSmall] > 0) {
     auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
-                              kWaves * 64, 0, l.cus, l.grid_cap);
+                              kWaves * 64, 0, l.cus, l.grid_cap,
+                              waves_to_blocks(p.ngroups[kTcSmall], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcSmall], p.groups[kTcSmall],
                        p.ngroups[kTcSmall], p.partners, p.queues + kTcSmall,
@@ -187,7 +197,8 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
   if (p.ngroups[kTcGlobal] > 0) {
     auto *kern = &tc_global_search<kWaves, ITEMS>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
-                              kWaves * 64, 0, l.cus, l.grid_cap);
+                              kWaves * 64, 0, l.cus, l.grid_cap,
+                              waves_to_blocks(p.ngroups[kTcGlobal], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcGlobal], p.groups[kTcGlobal],
                        p.ngroups[kTcGlobal], p.partners, p.queues + kTcGlobal,
@@ -196,7 +207,21 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+__global__ void zero_words(uint32_t *p, size_t n) {
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += size_t(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
 }  // namespace
+
+int device_zero(void *p, size_t words, hipStream_t stream) {
+  if (words == 0) return 0;
+  const size_t blocks = std::min<size_t>((words + 255) / 256, 1024);
+  hipLaunchKernelGGL(zero_words, dim3(static_cast<unsigned>(blocks)), dim3(256),
+                     0, stream, static_cast<uint32_t *>(p), words);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int tc_device_init() {
   const int lds = static_cast<int>(sizeof(int32_t) << kTcBlockLogpMax);
@@ -212,8 +237,9 @@ int tc_device_init() {
 
 int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
                      hipStream_t stream) {
-  hipLaunchKernelGGL(tc_reset, dim3(1), dim3(64), 0, stream, p.queues,
-                     kTcNumClasses, p.result);
+  if (l.reset)
+    hipLaunchKernelGGL(tc_reset, dim3(1), dim3(64), 0, stream, p.queues,
+                       kTcNumClasses, p.result);
   return l.items == 8 ? launch_all<8>(p, l, stream)
                       : launch_all<4>(p, l, stream);
 }

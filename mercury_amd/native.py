@@ -44,6 +44,37 @@ class Stats(C.Structure):
                 if name != "reserved"}
 
 
+class Query(C.Structure):
+    _fields_ = [("graph", C.c_void_p), ("pattern", C.c_int32),
+                ("status", C.c_int32), ("count", C.c_uint64)]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32),
+                ("max_batch", C.c_int32), ("lanes", C.c_int32),
+                ("hbm_budget", C.c_uint64), ("reserved", C.c_uint64 * 2)]
+
+
+class Request(C.Structure):
+    _fields_ = [("graph", C.c_int32), ("pattern", C.c_int32),
+                ("arrival_us", C.c_uint64), ("count", C.c_uint64),
+                ("status", C.c_int32), ("batch_size", C.c_int32),
+                ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64),
+                ("failed", C.c_uint64), ("launches", C.c_uint64),
+                ("resident_bytes", C.c_uint64),
+                ("workspace_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("busy_ms", C.c_double)]
+
+
 class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("max_degree", C.c_int32),
                 ("nnz", C.c_int64), ("meta_max_degree", C.c_int32),
@@ -83,6 +114,18 @@ def _load() -> C.CDLL:
                                     C.POINTER(Stats)]),
         "mgpm_pattern_count": (C.c_int, [P, C.c_int, C.POINTER(Tune), P,
                                          C.POINTER(u64), C.POINTER(Stats)]),
+        "mgpm_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchOpts), pp]),
+        "mgpm_batch_run": (C.c_int, [P, C.POINTER(Query), C.c_int, P,
+                                     C.POINTER(Stats)]),
+        "mgpm_batch_footprint": (C.c_int, [P, C.POINTER(u64)]),
+        "mgpm_batch_free": (C.c_int, [P]),
+        "mgpm_server_create": (C.c_int, [C.POINTER(ServerConfig), pp]),
+        "mgpm_server_add_graph": (C.c_int, [P, P, C.POINTER(i32)]),
+        "mgpm_server_submit": (C.c_int, [P, i32, i32, C.POINTER(u64)]),
+        "mgpm_server_wait": (C.c_int, [P, u64, i32, C.POINTER(Request)]),
+        "mgpm_server_replay": (C.c_int, [P, C.POINTER(Request), C.c_int]),
+        "mgpm_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+        "mgpm_server_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -101,7 +144,11 @@ EXPORTED = [
     "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
     "mgpm_dgraph_upload", "mgpm_dgraph_footprint", "mgpm_dgraph_free",
-    "mgpm_tc_count", "mgpm_pattern_count",
+    "mgpm_tc_count", "mgpm_pattern_count", "mgpm_batch_create",
+    "mgpm_batch_run", "mgpm_batch_footprint", "mgpm_batch_free",
+    "mgpm_server_create", "mgpm_server_add_graph", "mgpm_server_submit",
+    "mgpm_server_wait", "mgpm_server_replay", "mgpm_server_stats_get",
+    "mgpm_server_destroy",
 ]
 
 
@@ -274,6 +321,104 @@ class DeviceGraph:
     def close(self) -> None:
         if self._h:
             lib.mgpm_dgraph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many (graph, pattern) queries per call, overlapped over `lanes`
+    streams, optionally replayed from a recorded HIP graph
+    (``mgpm_batch``)."""
+
+    def __init__(self, device: int = 0, lanes: int = 0,
+                 use_graph: bool = False):
+        self._h = C.c_void_p()
+        opts = BatchOpts(lanes=lanes, use_graph=int(bool(use_graph)))
+        check(lib.mgpm_batch_create(device, C.byref(opts), C.byref(self._h)),
+              "mgpm_batch_create")
+
+    def run(self, queries, stream: Optional[int] = None):
+        """queries: list of (DeviceGraph, pattern).  Returns (counts, Stats)."""
+        arr = (Query * len(queries))()
+        for i, (dg, p) in enumerate(queries):
+            arr[i].graph = dg._h
+            arr[i].pattern = int(p)
+        st = Stats()
+        check(lib.mgpm_batch_run(self._h, arr, len(queries),
+                                 C.c_void_p(stream or 0), C.byref(st)),
+              "mgpm_batch_run")
+        return [arr[i].count for i in range(len(queries))], st
+
+    def footprint(self) -> int:
+        b = C.c_uint64()
+        check(lib.mgpm_batch_footprint(self._h, C.byref(b)), "footprint")
+        return b.value
+
+    def close(self) -> None:
+        if self._h:
+            lib.mgpm_batch_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching count server (``mgpm_server``)."""
+
+    def __init__(self, device: int = 0, slots: int = 0, max_batch: int = 0,
+                 hbm_budget: int = 0, lanes: int = 0):
+        cfg = ServerConfig(device=device, slots=slots, max_batch=max_batch,
+                           lanes=lanes, hbm_budget=hbm_budget)
+        self._h = C.c_void_p()
+        check(lib.mgpm_server_create(C.byref(cfg), C.byref(self._h)),
+              "mgpm_server_create")
+
+    def add_graph(self, graph: Graph) -> int:
+        gid = C.c_int32()
+        check(lib.mgpm_server_add_graph(self._h, graph._h, C.byref(gid)),
+              "mgpm_server_add_graph")
+        return gid.value
+
+    def submit(self, graph_id: int, pattern: int) -> int:
+        t = C.c_uint64()
+        check(lib.mgpm_server_submit(self._h, graph_id, pattern, C.byref(t)),
+              "mgpm_server_submit")
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> Request:
+        r = Request()
+        check(lib.mgpm_server_wait(self._h, ticket, timeout_ms, C.byref(r)),
+              "mgpm_server_wait")
+        return r
+
+    def replay(self, schedule):
+        """schedule: list of (graph_id, pattern, arrival_us).  Returns the
+        filled Request array."""
+        arr = (Request * len(schedule))()
+        for i, (g, p, t) in enumerate(schedule):
+            arr[i].graph, arr[i].pattern, arr[i].arrival_us = g, p, int(t)
+        rc = lib.mgpm_server_replay(self._h, arr, len(schedule))
+        if rc not in (0, -13):
+            check(rc, "mgpm_server_replay")
+        return arr
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        check(lib.mgpm_server_stats_get(self._h, C.byref(st)), "stats")
+        return st
+
+    def close(self) -> None:
+        if self._h:
+            lib.mgpm_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -49,6 +49,16 @@ def test_device_count_without_gpu_is_not_an_error():
     assert ma.device_count() >= 0
 
 
+@pytest.mark.skipif(ma.device_count() > 0, reason="checks the no-GPU path")
+def test_device_calls_without_gpu_fail_cleanly():
+    g = ma.Graph.rmat(6, 4, seed=1)
+    for make in (lambda: ma.DeviceGraph(g, 0), lambda: ma.Batch(0),
+                 lambda: ma.Server(0)):
+        with pytest.raises(ma.MgpmError) as ei:
+            make()
+        assert ei.value.code == -6  # MGPM_ERR_NODEVICE
+
+
 # ----------------------------------------------------------------- loader
 
 def _copy_fixture(tmp_path, name="gnp40"):

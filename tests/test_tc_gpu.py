@@ -20,6 +20,7 @@ FORCE = {
                        block_max_deg=16384),
     "global_only": dict(small_max_deg=-1, wave_max_deg=-1, block_max_deg=-1),
     "tiny_chunks": dict(chunk_work=7),
+    "items4": dict(tc_items=4),
     "items8": dict(tc_items=8),
     "items8_global": dict(tc_items=8, small_max_deg=-1, wave_max_deg=-1,
                           block_max_deg=-1),
@@ -100,3 +101,18 @@ def test_tc_repeatable(gpu_available):
     dg = ma.DeviceGraph(g, 0)
     counts = {dg.tc_count()[0] for _ in range(10)}
     assert len(counts) == 1
+
+
+@pytest.mark.parametrize("items", [4, 8])
+def test_tc_long_lists_owner_cursor(gpu_available, items):
+    """Dense G(n, p): partner lists average well over 32 items, so the
+    streaming loop takes its per-lane owner-cursor path (not the start-mark
+    path) in every class it is forced into."""
+    g = ma.Graph.er(3000, 450000, seed=13)
+    rp, ci = g.csr()
+    want = oracle.tc(rp, ci)
+    for mode in ("default", "small_only", "medium_only", "block_only",
+                 "global_only", "tiny_chunks"):
+        kw = dict(FORCE[mode] or {})
+        kw["tc_items"] = items
+        assert _gpu_tc(g, ma.make_tune(**kw))[0] == want, mode
