@@ -88,6 +88,14 @@ typedef struct mgpm_tune {
                              of up to this cost */
   int32_t tc_items;       /* TC: list items per lane per step (4 or 8;
                              default 8) */
+  int32_t tc_cursor_min;  /* TC: 64-partner batches whose lists average at
+                             least this many items are walked with one
+                             owner cursor per lane instead of start marks
+                             (default -1 = never) */
+  int32_t pattern_split;  /* P1..P8: a task (edge, or P1 vertex) whose
+                             estimated pair tests exceed 2^pattern_split is
+                             cut into shares run by different waves
+                             (default 20; -1 = never) */
 } mgpm_tune;
 
 /* Per-call measurements. */
@@ -183,6 +191,21 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
 int mgpm_pattern_count(mgpm_dgraph *dg, int pattern, const mgpm_tune *tune,
                        void *stream, uint64_t *count, mgpm_stats *stats);
 
+/* ---- autotuner --------------------------------------------------------------
+ * No counterpart in the reference, whose schedule constants are fixed
+ * (tuning_schedules.cuh:41-47, tc_challenge.cu:67-73).  MGPM_TC:
+ * coordinate descent over tc_items, tc_cursor_min, small/wave/block_max_deg
+ * (hash-table class vs binary search per owner size), chunk_work and
+ * group_work (task split and queue-entry packing); P1..P8: pattern_split
+ * and grid_blocks.  Each candidate is timed on
+ * the device (median kernel time of `reps` runs, 0 = 3) and must return the
+ * same count as the starting point (MGPM_ERR_INTERNAL otherwise).  The
+ * winner stays resident on the graph -- later calls with a NULL tune,
+ * batches and servers use it -- and is returned in *best (may be NULL)
+ * with its time in *best_ms (may be NULL). */
+int mgpm_autotune(mgpm_dgraph *dg, int pattern, int32_t reps, mgpm_tune *best,
+                  double *best_ms);
+
 /* ---- batches ------------------------------------------------------------
  * No counterpart in the reference (one graph, one pattern per process).
  * Many (graph, pattern) queries run in one call with no host round trip per
@@ -206,7 +229,11 @@ typedef struct mgpm_query {
 typedef struct mgpm_batch_opts {
   int32_t lanes;      /* concurrent streams (0 = 4, at most 16) */
   int32_t use_graph;  /* 1 = record and replay a HIP graph */
-  int32_t reserved[6];
+  int32_t autotune;   /* 1 = before the first run of a query set, tune the
+                         triangle knobs of the largest untuned graph of each
+                         size class (log2 of DAG arcs) with mgpm_autotune's
+                         search and apply them to the rest of the class */
+  int32_t reserved[5];
 } mgpm_batch_opts;
 
 /* opts may be NULL (defaults). */

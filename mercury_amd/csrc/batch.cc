@@ -17,13 +17,6 @@ namespace {
 // device but kept under this many bytes.
 constexpr uint64_t kLaneSlabBudget = uint64_t(2) << 30;
 
-// Relative weight of one unit of plan cost.  The per-edge proxy is
-// quadratic; the patterns whose set algebra nests another level over it
-// weigh more so that they start early on their lane.
-uint64_t pattern_weight(int p) {
-  return (p == MGPM_P1 || p == MGPM_P4) ? 1 : 4;
-}
-
 uint64_t slab_bytes(int64_t cap, int grid) {
   return uint64_t(pattern_waves_per_block()) * 8 * uint64_t(cap) *
          sizeof(int32_t) * uint64_t(grid);
@@ -33,11 +26,13 @@ uint64_t slab_bytes(int64_t cap, int grid) {
 
 uint64_t query_cost(const mgpm_dgraph *g, int pattern) {
   if (pattern == MGPM_TC) return g->tc.probes + g->tc.m_plus + 1;
+  // The pattern plan's costs bound the pair tests of the cubic patterns;
+  // the quadratic ones (P1, P4) are overestimated and simply start early.
   uint64_t c = 0;
   for (uint64_t x : g->pat.edge_group_cost) c += x;
   if (pattern == MGPM_P1)
     for (uint64_t x : g->pat.vert_group_cost) c += x;
-  return c * pattern_weight(pattern) + 1;
+  return c + 1;
 }
 
 BatchExec::~BatchExec() {
@@ -170,6 +165,7 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
         tl.cus = cus_;
         tl.items = qp.g->tc.items;
         tl.grid_cap = qp.g->tc.grid_cap;
+        tl.cursor_min = qp.g->tc.cursor_min;
         tl.reset = false;
         if (tc_device_launch(p, tl, ls))
           return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
@@ -179,7 +175,9 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
         p.cap = slab_cap_[l];
         p.queues = queues + 4 * i;
         p.counters = counters + 2 * i;
-        if (pattern_device_launch(p, qp.pattern, slab_grid_[l], ls, false))
+        int grid = slab_grid_[l];
+        if (qp.g->pat.grid_cap > 0) grid = std::min(grid, qp.g->pat.grid_cap);
+        if (pattern_device_launch(p, qp.pattern, grid, ls, false))
           return fail(MGPM_ERR_HIP, "pattern kernel launch failed");
       }
     }
@@ -308,11 +306,20 @@ int mgpm_batch_run(mgpm_batch *b, mgpm_query *queries, int nq, void *stream,
     double prep = 0;
     if (k != b->key) {
       const auto tp = std::chrono::steady_clock::now();
+      if (b->opts.autotune) {
+        std::vector<mgpm_dgraph *> tc;
+        for (int i = 0; i < nq; ++i)
+          if (queries[i].pattern == MGPM_TC && queries[i].graph &&
+              queries[i].graph->device == b->device)
+            tc.push_back(queries[i].graph);
+        if ((rc = autotune_size_classes(tc, 3))) return rc;
+        k = key_of(queries, nq);
+      }
       b->key.clear();
       drop_graph(b);
       if ((rc = b->exec.plan(queries, nq, UINT64_MAX))) return rc;
       if (b->opts.use_graph && (rc = record(b))) return rc;
-      b->key = std::move(k);
+      b->key = key_of(queries, nq);  // after any plan built on the way
       prep = ms_since(tp);
     }
     hipStream_t s = static_cast<hipStream_t>(stream);

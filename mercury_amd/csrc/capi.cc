@@ -391,6 +391,8 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   l.items = (tune && (tune->tc_items == 4 || tune->tc_items == 8))
                 ? tune->tc_items
                 : dg->tc.items;
+  l.cursor_min = (tune && tune->tc_cursor_min != 0) ? tune->tc_cursor_min
+                                                    : dg->tc.cursor_min;
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (tc_device_launch(dg->tc.dev, l, s))
     return fail(MGPM_ERR_HIP, "triangle kernel launch failed");

@@ -66,9 +66,12 @@ struct DevBuf {
 // Triangle-count state resident on one device.
 struct TcResident {
   TcPlanParams params;
-  uint64_t generation = 0;  // bumped whenever the plan or launch changes
+  uint64_t generation = 0;  // bumped whenever any plan of the graph (TC or
+                            // pattern) or its launch settings change
   int items = 8;            // list items per lane per step
   int grid_cap = 0;         // cap on persistent workgroups (0 = none)
+  int cursor_min = -1;      // owner-cursor streaming threshold (<= 0: off)
+  bool tuned = false;       // knobs chosen by the autotuner
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
   int32_t max_out = 0;
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
@@ -86,7 +89,9 @@ struct CsrResident {
 // single-query path, allocated on its first call.
 struct PatternResident {
   bool ready = false;
-  int grid = 0;
+  int grid = 0;      // workgroups the single-query slab holds
+  int grid_cap = 0;  // cap on persistent workgroups (0 = none)
+  int split = 20;    // shares above 2^split pair tests (-1: no splits)
   int32_t max_degree = 0;
   DevBuf edges, edge_groups, verts, vert_groups, slab;
   std::vector<uint64_t> edge_group_cost, vert_group_cost;  // host copies
@@ -102,8 +107,14 @@ TcPlanParams tc_params_from(const mgpm_tune *t);
 int set_device(int device);
 int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s);
-int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s);
+// Builds the pattern plan if missing or if `split` (0 = keep the current
+// setting) differs from the resident plan's.
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int split = 0);
 int pattern_make_slab(mgpm_dgraph *dg);
+// Tunes the triangle knobs of the largest not-yet-tuned graph of every
+// size class (log2 of DAG arcs) among `graphs` and applies the winner to
+// the others of its class.
+int autotune_size_classes(const std::vector<mgpm_dgraph *> &graphs, int reps);
 
 }  // namespace mgpm
 

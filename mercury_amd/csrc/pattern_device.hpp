@@ -10,14 +10,16 @@ namespace mgpm {
 struct PatternDevicePlan {
   const int64_t *rowptr = nullptr;  // undirected CSR
   const int32_t *col = nullptr;
-  const int2 *edges = nullptr;      // task edges (v0, v1), cost-ordered
+  const int4 *edges = nullptr;      // task-edge shares (v0, v1, sub, nsub),
+                                    // cost-ordered
   const int32_t *edge_groups = nullptr;
   int n_edge_groups = 0;
-  const int32_t *verts = nullptr;   // P1 vertex tasks, cost-ordered
+  const int4 *verts = nullptr;      // P1 vertex shares (v, -, sub, nsub)
   const int32_t *vert_groups = nullptr;
   int n_vert_groups = 0;
   int32_t *slab = nullptr;          // per-wave scratch, 8 * cap ints each
   int64_t cap = 0;                  // >= max degree
+  int32_t split = 62;               // shares per 2^split pair tests
   unsigned *queues = nullptr;       // 2 counters (edges, vertices)
   unsigned long long *counters = nullptr;  // c1, c2
 };

@@ -1,6 +1,5 @@
-// P1..P8 device functions shared by the per-graph kernels
-// (pattern_kernels.hip) and the batched multi-graph kernel
-// (batch_kernels.hip).  See pattern_kernels.hip for the design.
+// P1..P8 device functions of the pattern kernels (pattern_kernels.hip,
+// which describes the design).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -83,11 +82,15 @@ enum PairMode : int {
 
 // Number of pairs (x in X, y in Y) satisfying MODE; per-lane partial.
 // The outer loop runs over the shorter list (uniform element, its
-// adjacency list searched by all lanes), the lanes over the longer one.
+// adjacency list searched by all lanes), the lanes over the longer one;
+// only outer indices sub, sub + nsub, ... are taken (one share of a split
+// task).
 template <int MODE>
 __device__ __forceinline__ uint64_t pair_count(const PGraph &g,
                                                const int32_t *X, int32_t nx,
-                                               const int32_t *Y, int32_t ny) {
+                                               const int32_t *Y, int32_t ny,
+                                               int32_t sub = 0,
+                                               int32_t nsub = 1) {
   const int lane = lane64();
   uint64_t c = 0;
   if (nx == 0 || ny == 0) return 0;
@@ -95,7 +98,7 @@ __device__ __forceinline__ uint64_t pair_count(const PGraph &g,
   const int32_t *O = outer_x ? X : Y;
   const int32_t *I = outer_x ? Y : X;
   const int32_t no = outer_x ? nx : ny, ni = outer_x ? ny : nx;
-  for (int32_t a = 0; a < no; ++a) {
+  for (int32_t a = sub; a < no; a += nsub) {
     const int32_t o = O[a];
     const int32_t *No = g.nb(o);
     const int32_t dO = g.deg(o);
@@ -167,16 +170,42 @@ __device__ __forceinline__ uint32_t count_prefix(const PGraph &g, int32_t v,
   return c;
 }
 
+// Number of x in L[0, n) not adjacent to v (wave-uniform total).
+__device__ __forceinline__ uint32_t count_nonadj(const PGraph &g, int32_t v,
+                                                 const int32_t *L, int32_t n) {
+  return static_cast<uint32_t>(
+      wave_sum_i32(static_cast<int>(count_prefix<false>(g, v, L, n))));
+}
+
 // ------------------------------------------------------------ P1 vertex
 
 // P1_frequency_count (pattern_kernels.cuh:68-133) for vertex v0:
-// f[i] = #{j > i : N0[i] !~ N0[j]}, cnt[j] = #{i < j : N0[i] !~ N0[j]},
-// c1 += sum_k cnt[k] f[k].
-__device__ void p1_vertex(const PGraph &g, int32_t v0, Slab &s,
-                          uint64_t &c1) {
+// c1 += sum_k cnt[k] f[k] with cnt[k] = #{i < k : N0[i] !~ N0[k]} and
+// f[k] = #{j > k : N0[k] !~ N0[j]}.  Unsplit (nsub == 1): one pass over
+// the pairs i < j, f kept lane-owned.  Split: share `sub` takes the k with
+// k = sub (mod nsub) and gets both terms of each from one pass over N0.
+__device__ void p1_vertex(const PGraph &g, int32_t v0, int32_t sub,
+                          int32_t nsub, Slab &s, uint64_t &c1) {
   const int lane = lane64();
   const int32_t d0 = g.deg(v0);
   const int32_t *N0 = g.nb(v0);
+  if (nsub > 1) {
+    for (int32_t k = sub; k < d0; k += nsub) {
+      const int32_t y = N0[k];
+      const int32_t *Ny = g.nb(y);
+      const int32_t dy = g.deg(y);
+      int before = 0, after = 0;
+      for (int32_t i = lane; i < d0; i += 64) {
+        const bool non = i != k && !bs_contains(Ny, dy, N0[i]);
+        before += non && i < k;
+        after += non && i > k;
+      }
+      before = wave_sum_i32(before);
+      after = wave_sum_i32(after);
+      if (lane == 0) c1 += static_cast<uint64_t>(before) * uint32_t(after);
+    }
+    return;
+  }
   zero_lane_owned(s.F, d0);
   for (int32_t j = 0; j < d0; ++j) {
     const int32_t y = N0[j];
@@ -199,39 +228,64 @@ __device__ void p1_vertex(const PGraph &g, int32_t v0, Slab &s,
 
 // -------------------------------------------------------------- edges
 
+// Shares a quadratic task (P1's correction, P4: at most d0 * d1 pair
+// tests) keeps of the `nsub` the plan cut for the cubic patterns: one per
+// 2^split pair tests.
+__device__ __forceinline__ int32_t quadratic_shares(int32_t d0, int32_t d1,
+                                                    int32_t split,
+                                                    int32_t nsub) {
+  const uint64_t work = uint64_t(d0) * uint64_t(d1);
+  const uint64_t want = (work >> split) + 1;
+  return want < uint64_t(nsub) ? static_cast<int32_t>(want) : nsub;
+}
+
+// One share of the task edge (v0, v1): outer loops take the indices
+// sub, sub + nsub, ...; nsub == 1 is the whole edge.  Per-index frequency
+// factors that the reference accumulates over the outer loop are computed
+// per index here, so shares of one edge need no common state.  The plan's
+// split exponent `split` trims the share count of the quadratic patterns
+// (shares past it do nothing).
 template <int P>
-__device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1, Slab &s,
+__device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
+                          int32_t sub, int32_t nsub, int32_t split, Slab &s,
                           uint64_t &c1, uint64_t &c2) {
   const int lane = lane64();
   const int32_t *N0 = g.nb(v0), *N1 = g.nb(v1);
   const int32_t d0 = g.deg(v0), d1 = g.deg(v1);
   int32_t **L = s.L;
+  if constexpr (P == 1 || P == 4) {
+    if (nsub > 1) {
+      const int32_t nq = quadratic_shares(d0, d1, split, nsub);
+      if (sub >= nq) return;
+      nsub = nq;
+    }
+  }
   if constexpr (P == 1) {  // P1_count_correction :6-66
     if (v0 == v1) return;
     const int32_t na = wave_diff(N0, d0, N1, d1, v1, L[0]);
     const int32_t nb = wave_inter(N0, d0, N1, d1, v1, L[1]);
     wave_sync();
-    c2 += pair_count<kNonAdjYGt>(g, L[1], nb, L[0], na);
+    c2 += pair_count<kNonAdjYGt>(g, L[1], nb, L[0], na, sub, nsub);
   } else if constexpr (P == 2 || P == 8) {  // :135-243, :461-572
     const int32_t n0 = P == 2 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
                               : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
-    zero_lane_owned(s.F, n0);
     wave_sync();
-    for (int32_t i = 0; i < n1; ++i) {
+    for (int32_t i = sub; i < n1; i += nsub) {
       const int32_t v4 = L[1][i];
       int32_t n2, n3;
-      // L2 = D(L0, N(v4)), L3 = I(L0, N(v4)); f[j] += L0[j] !~ v4
-      split_by(g, v4, L[0], n0, L[2], n2, L[3], n3, s.F);
+      // L2 = D(L0, N(v4)), L3 = I(L0, N(v4))
+      split_by(g, v4, L[0], n0, L[2], n2, L[3], n3, nullptr);
       wave_sync();
       c2 += P == 2 ? pair_count<kNonAdjYGt>(g, L[3], n3, L[2], n2)
                    : pair_count<kAdjYGt>(g, L[3], n3, L[2], n2);
       wave_sync();
     }
-    // c1 += |D<(L0,N(y),y)| f[i]  (P2)   or  |I<(L0,N(y),y)| f[i]  (P8)
-    for (int32_t i = 0; i < n0; ++i) {
+    // c1 += |D<(L0,N(y),y)| f  (P2)  or  |I<(L0,N(y),y)| f  (P8), with
+    // f = #{v4 in L1 : y !~ v4}
+    for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t y = L[0][i];
-      const uint32_t f = s.F[i];
+      const uint32_t f = count_nonadj(g, y, L[1], n1);
       if (f == 0) continue;
       const uint32_t part = P == 2 ? count_prefix<false>(g, y, L[0], i)
                                    : count_prefix<true>(g, y, L[0], i);
@@ -240,62 +294,61 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1, Slab &s,
   } else if constexpr (P == 3 || P == 7) {  // :245-350, :352-459
     const int32_t n0 = P == 3 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
                               : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
-    zero_lane_owned(s.F, n0);
     wave_sync();
-    for (int32_t i = 0; i < n0; ++i) {
+    for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t y = L[0][i];
       const int32_t *Ny = g.nb(y);
       const int32_t dy = g.deg(y);
       // A = D<(L0, N(y), y) (indices < i, not adjacent), B = I(L0, N(y));
-      // f[j] += 1 for j in A; cnt = |A|.
-      int32_t na = 0, nb = 0;
+      // c2 over pairs of B x A; c1 += |A| * #{j > i : L0[j] !~ y}.
+      int32_t na = 0, nb = 0, nafter = 0;
       for (int32_t c = 0; c < n0; c += 64) {
         const int32_t j = c + lane;
         const bool valid = j < n0;
         const int32_t x = valid ? L[0][j] : 0;
         const bool adj = valid && bs_contains(Ny, dy, x);
         const bool inA = valid && j < i && !adj;
-        if (inA) s.F[j] += 1;
         const unsigned long long ma = __ballot(inA), mb = __ballot(adj);
+        const unsigned long long mf = __ballot(valid && j > i && !adj);
         if (inA) L[1][na + bits_below(ma)] = x;
         if (adj) L[2][nb + bits_below(mb)] = x;
         na += __popcll(ma);
         nb += __popcll(mb);
+        nafter += __popcll(mf);
       }
-      if (lane == 0) s.CNT[i] = static_cast<uint32_t>(na);
+      if (lane == 0) c1 += static_cast<uint64_t>(na) * uint32_t(nafter);
       wave_sync();
       c2 += pair_count<kNonAdjYGt>(g, L[2], nb, L[1], na);
       wave_sync();
     }
-    for (int32_t k = lane; k < n0; k += 64)
-      c1 += static_cast<uint64_t>(s.CNT[k]) * s.F[k];
   } else if constexpr (P == 5) {  // :574-734
     const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N0, d0, N1, d1, kNoUb, L[1]);
     const int32_t n2 = wave_diff(N1, d1, N0, d0, kNoUb, L[2]);
-    zero_lane_owned(s.F, n1);
     wave_sync();
-    for (int32_t i = 0; i < n0; ++i) {
+    for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t v2 = L[0][i];
       int32_t n4, n5, unused;
-      // L4 = D(L1, N(v2)) with f[j] += L1[j] !~ v2; L5 = I(L2, N(v2))
-      split_by(g, v2, L[1], n1, L[4], n4, nullptr, unused, s.F);
+      // L4 = D(L1, N(v2)); L5 = I(L2, N(v2))
+      split_by(g, v2, L[1], n1, L[4], n4, nullptr, unused, nullptr);
       split_by(g, v2, L[2], n2, nullptr, unused, L[5], n5, nullptr);
       wave_sync();
       c2 += pair_count<kNonAdj>(g, L[5], n5, L[4], n4);
       wave_sync();
     }
-    for (int32_t i = 0; i < n1; ++i) {
-      const uint32_t f = s.F[i];
+    // c1 += f * |D(L2, N(x))| for x = L1[i], f = #{v2 in L0 : x !~ v2}
+    for (int32_t i = sub; i < n1; i += nsub) {
+      const int32_t x = L[1][i];
+      const uint32_t f = count_nonadj(g, x, L[0], n0);
       if (f == 0) continue;
-      const uint32_t t = count_prefix<false>(g, L[1][i], L[2], n2);
+      const uint32_t t = count_prefix<false>(g, x, L[2], n2);
       c1 += static_cast<uint64_t>(t) * f;
     }
   } else if constexpr (P == 6) {  // :736-853
     const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
     wave_sync();
-    for (int32_t i = 0; i < n0; ++i) {
+    for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t c = L[0][i];
       int32_t n2, n3, unused;
       split_by(g, c, L[0], n0, L[2], n2, nullptr, unused, nullptr);
@@ -309,7 +362,7 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1, Slab &s,
     const int32_t n0 = wave_diff(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
     wave_sync();
-    c1 += pair_count<kNonAdj>(g, L[0], n0, L[1], n1);
+    c1 += pair_count<kNonAdj>(g, L[0], n0, L[1], n1, sub, nsub);
   }
   wave_sync();
 }

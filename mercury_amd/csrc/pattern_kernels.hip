@@ -8,13 +8,17 @@
 //   * one wave per task edge (P1 also per vertex), tasks taken from a
 //     host-built cost-ordered queue of task groups instead of edge-id
 //     striding, so hub edges start first and light edges are batched;
+//   * heavy edges and vertices are split into shares, each its own queue
+//     entry: a share runs the task's outer loops over every nsub-th index,
+//     and the frequency factors the reference accumulates across the outer
+//     loop (f[] of P2/P3/P5/P7/P8) are recounted per index instead, so one
+//     hub edge is spread over many waves with no shared state;
 //   * frontier lists are built with 64-lane ballot + mbcnt compaction;
 //   * every "x in N(y)" test is an exact binary search, done by 64 lanes at
 //     once over one uniform list N(y) (good locality), with the outer loop
 //     over the shorter of two frontier lists;
-//   * frequency arrays are lane-owned (index i always handled by lane
-//     i mod 64), so they need no atomics, and the per-index counts the
-//     reference recomputes in a second pass come out of the same pass;
+//   * P1's unsplit vertex pass keeps its frequency array lane-owned
+//     (index i always handled by lane i mod 64), so it needs no atomics;
 //   * counts are exact uint64 everywhere (the reference narrows
 //     warp_reduce results and their products to int32).
 // Counters: counters[0] = the reference's counters[1] (c1), counters[1] =
@@ -36,7 +40,30 @@ constexpr int kPWaves = 4;
 // One wave per task edge; queue entries are groups of task edges.
 template <int P>
 __global__ __launch_bounds__(kPWaves * 64) void pattern_edges(
-    PGraph g, const int2 *__restrict__ edges,
+    PGraph g, const int4 *__restrict__ edges,
+    const int32_t *__restrict__ goff, int ngroups, int32_t *slab,
+    int64_t cap, int split, unsigned *queue, unsigned long long *counters) {
+  const int lane = lane64();
+  const int64_t wave = int64_t(blockIdx.x) * kPWaves + (threadIdx.x >> 6);
+  Slab s(slab + wave * 8 * cap, cap);
+  uint64_t c1 = 0, c2 = 0;
+  for (;;) {
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(queue, 1u);
+    q = wave_first(q);
+    if (q >= static_cast<uint32_t>(ngroups)) break;
+    const int t_end = goff[q + 1];
+    for (int t = goff[q]; t < t_end; ++t) {
+      const int4 e = edges[t];
+      edge_task<P>(g, e.x, e.y, e.z, e.w, split, s, c1, c2);
+    }
+  }
+  flush2(c1, c2, counters);
+}
+
+// P1's vertex pass (frequency count over every vertex).
+__global__ __launch_bounds__(kPWaves * 64) void pattern_p1_vertices(
+    PGraph g, const int4 *__restrict__ verts,
     const int32_t *__restrict__ goff, int ngroups, int32_t *slab,
     int64_t cap, unsigned *queue, unsigned long long *counters) {
   const int lane = lane64();
@@ -50,29 +77,9 @@ __global__ __launch_bounds__(kPWaves * 64) void pattern_edges(
     if (q >= static_cast<uint32_t>(ngroups)) break;
     const int t_end = goff[q + 1];
     for (int t = goff[q]; t < t_end; ++t) {
-      const int2 e = edges[t];
-      edge_task<P>(g, e.x, e.y, s, c1, c2);
+      const int4 v = verts[t];
+      p1_vertex(g, v.x, v.z, v.w, s, c1);
     }
-  }
-  flush2(c1, c2, counters);
-}
-
-// P1's vertex pass (frequency count over every vertex).
-__global__ __launch_bounds__(kPWaves * 64) void pattern_p1_vertices(
-    PGraph g, const int32_t *__restrict__ verts,
-    const int32_t *__restrict__ goff, int ngroups, int32_t *slab,
-    int64_t cap, unsigned *queue, unsigned long long *counters) {
-  const int lane = lane64();
-  const int64_t wave = int64_t(blockIdx.x) * kPWaves + (threadIdx.x >> 6);
-  Slab s(slab + wave * 8 * cap, cap);
-  uint64_t c1 = 0, c2 = 0;
-  for (;;) {
-    uint32_t q = 0;
-    if (lane == 0) q = atomicAdd(queue, 1u);
-    q = wave_first(q);
-    if (q >= static_cast<uint32_t>(ngroups)) break;
-    const int t_end = goff[q + 1];
-    for (int t = goff[q]; t < t_end; ++t) p1_vertex(g, verts[t], s, c1);
   }
   flush2(c1, c2, counters);
 }
@@ -88,7 +95,7 @@ void launch_edges(const PatternDevicePlan &p, int grid, hipStream_t s) {
   PGraph g{p.rowptr, p.col};
   hipLaunchKernelGGL(pattern_edges<P>, dim3(grid), dim3(kPWaves * 64), 0, s,
                      g, p.edges, p.edge_groups, p.n_edge_groups, p.slab,
-                     p.cap, p.queues, p.counters);
+                     p.cap, p.split, p.queues, p.counters);
 }
 
 // Workgroups that can take a queue entry: one entry per wave.

@@ -59,27 +59,83 @@ void order_and_group(int64_t count, Cost cost_of, const std::vector<Item> &in,
 
 }  // namespace
 
-void build_pattern_plan(const CsrGraph &g, PatternPlan &plan) {
+namespace {
+
+// Shares of a task of `work` estimated pair tests whose outer loops run
+// over at most `outer` indices.
+int32_t shares(uint64_t work, int64_t outer, uint64_t split_work) {
+  const uint64_t want = (work + split_work - 1) / split_work;
+  const int64_t cap =
+      std::min<int64_t>(kPatternMaxSplit, std::max<int64_t>(1, outer));
+  return static_cast<int32_t>(
+      std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(want), cap)));
+}
+
+}  // namespace
+
+void build_pattern_plan(const CsrGraph &g, PatternPlan &plan,
+                        uint64_t split_work) {
+  if (split_work == 0) split_work = kPatternSplitWork;
   const int32_t n = g.n;
   plan.max_degree = g.max_degree();
-  std::vector<PatternTaskEdge> coo(static_cast<size_t>(g.nnz));
+  // Per directed edge: the pair tests of the heaviest patterns, bounded by
+  // d0 * d1 * min(d0, d1) / 4, and the outer-loop length min(d0, d1).  That
+  // bound (per share) also orders and groups the tasks.
+  auto edge_work = [&](int32_t v, int32_t u) {
+    const uint64_t d0 = g.degree(v) + 1, d1 = g.degree(u) + 1;
+    return d0 * d1 * std::min(d0, d1) / 4 + 1;
+  };
+  std::vector<int32_t> esplit(static_cast<size_t>(g.nnz));
+  std::vector<int64_t> eoff(static_cast<size_t>(n) + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1024)
-  for (int32_t v = 0; v < n; ++v)
-    for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e)
-      coo[e] = PatternTaskEdge{v, g.colidx[e]};
-  // Quadratic proxy of the per-edge set algebra: (d0 + 16) * (d1 + 16).
+  for (int32_t v = 0; v < n; ++v) {
+    int64_t acc = 0;
+    for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e) {
+      const int32_t u = g.colidx[e];
+      const int64_t lo = std::min(g.degree(v), g.degree(u));
+      esplit[e] = shares(edge_work(v, u), lo, split_work);
+      acc += esplit[e];
+    }
+    eoff[v + 1] = acc;
+  }
+  for (int32_t v = 0; v < n; ++v) eoff[v + 1] += eoff[v];
+  std::vector<PatternTask> tasks(static_cast<size_t>(eoff[n]));
+  std::vector<uint64_t> tcost(tasks.size());
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int32_t v = 0; v < n; ++v) {
+    int64_t k = eoff[v];
+    for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e) {
+      const int32_t u = g.colidx[e];
+      // Shares of a split edge each get a queue entry of their own.
+      const uint64_t w = edge_work(v, u) / uint64_t(esplit[e]) + 1;
+      const uint64_t share = esplit[e] > 1 ? std::max(w, kGroupCost) : w;
+      for (int32_t s = 0; s < esplit[e]; ++s, ++k) {
+        tasks[k] = PatternTask{v, u, s, esplit[e]};
+        tcost[k] = share;
+      }
+    }
+  }
   order_and_group(
-      g.nnz,
-      [&](int64_t i) {
-        return uint64_t(g.degree(coo[i].v0) + 16) *
-               uint64_t(g.degree(coo[i].v1) + 16);
-      },
-      coo, plan.edges, plan.edge_groups, plan.edge_group_cost);
-  std::vector<int32_t> ids(static_cast<size_t>(n));
-  for (int32_t v = 0; v < n; ++v) ids[v] = v;
+      static_cast<int64_t>(tasks.size()), [&](int64_t i) { return tcost[i]; },
+      tasks, plan.edges, plan.edge_groups, plan.edge_group_cost);
+  // P1 vertices: d(d-1)/2 pair tests, outer loop over d.
+  std::vector<PatternTask> vt;
+  std::vector<uint64_t> vcost;
+  vt.reserve(static_cast<size_t>(n));
+  for (int32_t v = 0; v < n; ++v) {
+    const uint64_t d = g.degree(v);
+    const uint64_t work = (d + 1) * (d + 1) / 2 + 1;
+    const int32_t ns = shares(work, static_cast<int64_t>(d), split_work);
+    const uint64_t w = work / uint64_t(ns) + 1;
+    const uint64_t share = ns > 1 ? std::max(w, kGroupCost) : w;
+    for (int32_t s = 0; s < ns; ++s) {
+      vt.push_back(PatternTask{v, 0, s, ns});
+      vcost.push_back(share);
+    }
+  }
   order_and_group(
-      n, [&](int64_t v) { return uint64_t(g.degree(v) + 16) * (g.degree(v) + 16); },
-      ids, plan.verts, plan.vert_groups, plan.vert_group_cost);
+      static_cast<int64_t>(vt.size()), [&](int64_t i) { return vcost[i]; },
+      vt, plan.verts, plan.vert_groups, plan.vert_group_cost);
 }
 
 }  // namespace mgpm

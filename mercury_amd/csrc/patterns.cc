@@ -25,11 +25,14 @@ int upload(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
-int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int split) {
   PatternResident &r = dg->pat;
-  if (r.ready) return MGPM_OK;
+  if (split == 0) split = r.split;
+  if (r.ready && split == r.split) return MGPM_OK;
+  const bool replace = r.ready;
   PatternPlan plan;
-  build_pattern_plan(*dg->host, plan);
+  build_pattern_plan(*dg->host, plan,
+                     split < 0 ? UINT64_MAX : uint64_t(1) << std::min(split, 62));
   int rc;
   if ((rc = upload(r.edges, plan.edges, s))) return rc;
   if ((rc = upload(r.edge_groups, plan.edge_groups, s))) return rc;
@@ -40,19 +43,21 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s) {
   PatternDevicePlan &d = r.dev;
   d.rowptr = dg->csr.rowptr.as<int64_t>();
   d.col = dg->csr.colidx.as<int32_t>();
-  d.edges = r.edges.as<int2>();
+  d.edges = r.edges.as<int4>();
   d.edge_groups = r.edge_groups.as<int32_t>();
   d.n_edge_groups = static_cast<int>(plan.edge_groups.size()) - 1;
-  d.verts = r.verts.as<int32_t>();
+  d.verts = r.verts.as<int4>();
   d.vert_groups = r.vert_groups.as<int32_t>();
   d.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
-  d.slab = nullptr;
+  d.slab = r.slab.as<int32_t>();  // kept across re-plans (same cap)
   d.cap = cap;
+  d.split = split < 0 ? 62 : std::min(split, 62);
   d.queues = reinterpret_cast<unsigned *>(dg->scratch.as<char>() + 128);
   d.counters =
       reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 192);
-  r.grid = 0;
+  r.split = split;
   r.max_degree = plan.max_degree;
+  if (replace) dg->tc.generation++;  // batches holding the old plan re-plan
   r.edge_group_cost = std::move(plan.edge_group_cost);
   r.vert_group_cost = std::move(plan.vert_group_cost);
   r.ready = true;
@@ -90,14 +95,20 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     double prep = 0;
-    if (!dg->pat.ready || !dg->pat.slab.p) {
+    const int split = tune ? tune->pattern_split : 0;
+    if (!dg->pat.ready || !dg->pat.slab.p ||
+        (split != 0 && split != dg->pat.split)) {
       const auto tp = std::chrono::steady_clock::now();
-      if ((rc = pattern_make_resident(dg, s))) return rc;
+      if ((rc = pattern_make_resident(dg, s, split))) return rc;
       if ((rc = pattern_make_slab(dg))) return rc;
       prep = ms_since(tp);
     }
+    int grid = dg->pat.grid;
+    const int cap = (tune && tune->grid_blocks > 0) ? tune->grid_blocks
+                                                    : dg->pat.grid_cap;
+    if (cap > 0) grid = std::min(grid, cap);
     MGPM_HIP(hipEventRecord(dg->ev0, s));
-    if (pattern_device_launch(dg->pat.dev, pattern, dg->pat.grid, s))
+    if (pattern_device_launch(dg->pat.dev, pattern, grid, s))
       return fail(MGPM_ERR_HIP, "pattern kernel launch failed");
     MGPM_HIP(hipEventRecord(dg->ev1, s));
     MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->pat.dev.counters, 16,

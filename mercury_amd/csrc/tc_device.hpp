@@ -33,6 +33,8 @@ struct TcLaunch {
   int grid_cap = 0;   // cap on persistent workgroups per kernel (0 = none)
   int items = 8;      // list items per lane per step (4 or 8)
   bool reset = true;  // zero queues and result first (else the caller has)
+  int cursor_min = -1;  // owner-cursor streaming from this average list
+                        // length of a 64-partner batch (<= 0: never)
 };
 
 // Enqueues reset + counting kernels on `stream`; the count lands in

@@ -1,6 +1,5 @@
-// Triangle-count device functions shared by the per-graph kernels
-// (tc_kernels.hip) and the batched multi-graph kernel (batch_kernels.hip).
-// See tc_kernels.hip for the design.
+// Triangle-count device functions of the counting kernels (tc_kernels.hip,
+// which describes the design).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,21 +11,19 @@ namespace mgpm {
 namespace tcdev {
 
 constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
-// Partner batches whose lists average at least this many items use the
-// per-lane owner cursor instead of start marks.
-constexpr int kCursorMinAvgLen = 32;
 
 // Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`.
 // Partners are taken 64 at a time (batches first_batch, first_batch +
 // batch_stride, ...); their lists are concatenated and walked 64*ITEMS
-// items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Returns
-// this lane's hit count.
+// items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Batches
+// whose lists average `cursor_min` items or more (never if <= 0) take the
+// owner-cursor path.  Returns this lane's hit count.
 template <int ITEMS, class Probe>
 __device__ __forceinline__ uint32_t stream_partners(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
     int first_batch, int batch_stride, int32_t *mark_lds,
-    const Probe &probe) {
+    const Probe &probe, int cursor_min) {
   // The start marks are written by one lane and read by another; a wave's
   // LDS instructions execute in order, so `volatile` (no compiler
   // reordering or store forwarding) is all the ordering needed.
@@ -45,7 +42,7 @@ __device__ __forceinline__ uint32_t stream_partners(
     const int incl = wave_incl_scan(len);
     const int excl = incl - len;
     const int total = __builtin_amdgcn_readlane(incl, 63);
-    if (total >= kCursorMinAvgLen * 64) {
+    if (cursor_min > 0 && total >= cursor_min * 64) {
       // Long lists on average: each lane walks its own owner cursor over
       // the batch's prefix sums (kept in LDS: a cursor advances in
       // divergent code, where cross-lane shuffles would read inactive
@@ -190,12 +187,13 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
                                               uint32_t ncol,
                                               const TcPartner *__restrict__ partners,
                                               const TcTask &task, int32_t *table,
-                                              int logp_max, int32_t *mark) {
+                                              int logp_max, int32_t *mark,
+                                              int cursor_min) {
   const int lane = lane64();
   if (2 * task.tlen > (1 << logp_max)) {
     SortedProbe probe{col + task.tbeg, task.tlen};
     return stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0,
-                                  1, mark, probe);
+                                  1, mark, probe, cursor_min);
   }
   int logp = ceil_log2(4 * task.tlen);
   logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
@@ -205,8 +203,8 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
   table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
   wave_sync();
   HashProbe probe{table, logp, static_cast<uint32_t>(P - 1)};
-  const uint32_t hits = stream_partners<ITEMS>(col, ncol, partners, task.pbeg,
-                                               task.pcnt, 0, 1, mark, probe);
+  const uint32_t hits = stream_partners<ITEMS>(
+      col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min);
   wave_sync();
   return hits;
 }

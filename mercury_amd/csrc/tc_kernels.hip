@@ -36,7 +36,8 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
-    unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
+    int cursor_min) {
   __shared__ int32_t s_table[WAVES][1 << LOGP_MAX];
   __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   const int lane = lane64();
@@ -51,7 +52,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     const int t_end = goff[g + 1];
     for (int t = goff[g]; t < t_end; ++t)
       acc += wave_task<ITEMS>(col, ncol, partners, tasks[t], table, LOGP_MAX,
-                              s_mark[wv]);
+                              s_mark[wv], cursor_min);
   }
   flush_hits(acc, result);
 }
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
-    int logp_max) {
+    int logp_max, int cursor_min) {
   extern __shared__ int32_t s_table[];
   __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   __shared__ uint32_t s_task;
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     if (2 * task.tlen > (1 << logp_max)) {  // never planned; kept exact
       SortedProbe probe{col + task.tbeg, task.tlen};
       acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
-                             WAVES, s_mark[wv], probe);
+                             WAVES, s_mark[wv], probe, cursor_min);
       __syncthreads();
       continue;
     }
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     __syncthreads();
     HashProbe probe{s_table, logp, static_cast<uint32_t>(P - 1)};
     acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
-                           WAVES, s_mark[wv], probe);
+                           WAVES, s_mark[wv], probe, cursor_min);
     __syncthreads();
     }
   }
@@ -107,7 +108,8 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
-    unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
+    int cursor_min) {
   __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   const int lane = lane64();
   const int wv = threadIdx.x >> 6;
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
       const TcTask task = tasks[t];
       SortedProbe probe{col + task.tbeg, task.tlen};
       acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
-                             s_mark[wv], probe);
+                             s_mark[wv], probe, cursor_min);
     }
   }
   flush_hits(acc, result);
@@ -172,7 +174,7 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlockWaves * 64), lds, stream,
                        p.col, p.ncol, p.tasks[kTcBlock], p.groups[kTcBlock],
                        p.ngroups[kTcBlock], p.partners, p.queues + kTcBlock,
-                       p.result, logp);
+                       p.result, logp, l.cursor_min);
   }
   if (p.ngroups[kTcMedium] > 0) {
     auto *kern = &tc_wave_tables<kMediumLogp, kWaves, ITEMS>;
@@ -182,7 +184,7 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcMedium], p.groups[kTcMedium],
                        p.ngroups[kTcMedium], p.partners,
-                       p.queues + kTcMedium, p.result);
+                       p.queues + kTcMedium, p.result, l.cursor_min);
   }
   if (p.ngroups[kTcSmall] > 0) {
     auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS>;
@@ -192,7 +194,7 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcSmall], p.groups[kTcSmall],
                        p.ngroups[kTcSmall], p.partners, p.queues + kTcSmall,
-                       p.result);
+                       p.result, l.cursor_min);
   }
   if (p.ngroups[kTcGlobal] > 0) {
     auto *kern = &tc_global_search<kWaves, ITEMS>;
@@ -202,7 +204,7 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcGlobal], p.groups[kTcGlobal],
                        p.ngroups[kTcGlobal], p.partners, p.queues + kTcGlobal,
-                       p.result);
+                       p.result, l.cursor_min);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

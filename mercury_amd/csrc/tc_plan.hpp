@@ -53,7 +53,7 @@ struct TcPlanParams {
   int32_t small_max = 128;
   int32_t wave_max = 1024;
   int32_t block_max = 16384;
-  int64_t chunk_work = 32768;  // max cost of one task (heavy owners split)
+  int64_t chunk_work = 65536;  // max cost of one task (heavy owners split)
   int64_t group_work = 32768;  // consecutive light tasks grouped up to this
 };
 

@@ -28,7 +28,8 @@ class Tune(C.Structure):
     _fields_ = [("small_max_deg", C.c_int32), ("wave_max_deg", C.c_int32),
                 ("block_max_deg", C.c_int32), ("chunk_work", C.c_int32),
                 ("grid_blocks", C.c_int32), ("pattern_waves", C.c_int32),
-                ("group_work", C.c_int32), ("tc_items", C.c_int32)]
+                ("group_work", C.c_int32), ("tc_items", C.c_int32),
+                ("tc_cursor_min", C.c_int32), ("pattern_split", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -51,7 +52,7 @@ class Query(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("autotune", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class ServerConfig(C.Structure):
@@ -114,6 +115,8 @@ def _load() -> C.CDLL:
                                     C.POINTER(Stats)]),
         "mgpm_pattern_count": (C.c_int, [P, C.c_int, C.POINTER(Tune), P,
                                          C.POINTER(u64), C.POINTER(Stats)]),
+        "mgpm_autotune": (C.c_int, [P, C.c_int, i32, C.POINTER(Tune),
+                                    C.POINTER(C.c_double)]),
         "mgpm_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchOpts), pp]),
         "mgpm_batch_run": (C.c_int, [P, C.POINTER(Query), C.c_int, P,
                                      C.POINTER(Stats)]),
@@ -144,7 +147,8 @@ EXPORTED = [
     "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
     "mgpm_dgraph_upload", "mgpm_dgraph_footprint", "mgpm_dgraph_free",
-    "mgpm_tc_count", "mgpm_pattern_count", "mgpm_batch_create",
+    "mgpm_tc_count", "mgpm_pattern_count", "mgpm_autotune",
+    "mgpm_batch_create",
     "mgpm_batch_run", "mgpm_batch_footprint", "mgpm_batch_free",
     "mgpm_server_create", "mgpm_server_add_graph", "mgpm_server_submit",
     "mgpm_server_wait", "mgpm_server_replay", "mgpm_server_stats_get",
@@ -318,6 +322,14 @@ class DeviceGraph:
                                      C.byref(st)), "mgpm_pattern_count")
         return cnt.value, st
 
+    def autotune(self, pattern: int = 0, reps: int = 0) -> tuple[Tune, float]:
+        """Times candidate knobs on the device, keeps the fastest resident
+        and returns it with its kernel time in ms."""
+        t, ms = Tune(), C.c_double()
+        check(lib.mgpm_autotune(self._h, int(pattern), reps, C.byref(t),
+                                C.byref(ms)), "mgpm_autotune")
+        return t, ms.value
+
     def close(self) -> None:
         if self._h:
             lib.mgpm_dgraph_free(self._h)
@@ -336,9 +348,10 @@ class Batch:
     (``mgpm_batch``)."""
 
     def __init__(self, device: int = 0, lanes: int = 0,
-                 use_graph: bool = False):
+                 use_graph: bool = False, autotune: bool = False):
         self._h = C.c_void_p()
-        opts = BatchOpts(lanes=lanes, use_graph=int(bool(use_graph)))
+        opts = BatchOpts(lanes=lanes, use_graph=int(bool(use_graph)),
+                         autotune=int(bool(autotune)))
         check(lib.mgpm_batch_create(device, C.byref(opts), C.byref(self._h)),
               "mgpm_batch_create")
 

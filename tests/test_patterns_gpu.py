@@ -108,3 +108,38 @@ def test_tc_challenge_cli_matches_golden(gpu_available):
     line = [l for l in r.stdout.splitlines()
             if l.startswith("total_num_triangles = ")][0]
     assert int(line.split("=")[1]) == want
+
+
+@pytest.mark.parametrize("split", [1, 6, 12, -1])
+def test_patterns_split_shares_vs_oracle(gpu_available, split):
+    """pattern_split forces heavy tasks into shares: at 2^1 nearly every edge
+    and vertex is cut into min(d0, d1) (or d) shares, at -1 none is.  Every
+    count must stay equal to the oracle's."""
+    for scale, ef, seed in [(8, 8, 31), (7, 16, 32)]:
+        g = ma.Graph.rmat(scale, ef, seed=seed)
+        rp, ci = g.csr()
+        dg = ma.DeviceGraph(g, 0)
+        t = ma.make_tune(pattern_split=split)
+        for p in PATTERNS:
+            want = oracle.pattern(rp, ci, p)[0]
+            got = dg.pattern_count(p, t)[0]
+            print(f"split {split} rmat{scale} P{p}: {got} vs {want}",
+                  flush=True)
+            assert got == want, (split, p)
+
+
+def test_patterns_split_hub_star_and_clique(gpu_available):
+    """A hub joined to a clique: the hub's edges are the heaviest tasks and
+    are split at the default threshold."""
+    n = 300
+    edges = [(0, i) for i in range(1, n)]
+    edges += [(a, b) for a in range(1, 60) for b in range(a + 1, 60)]
+    edges += [(i, i + 1) for i in range(60, n - 1)]
+    rp, ci = csr_from_edges(n, edges)
+    g = ma.Graph.from_csr(rp, ci)
+    dg = ma.DeviceGraph(g, 0)
+    for split in (0, 8, -1):
+        t = ma.make_tune(pattern_split=split) if split else None
+        for p in PATTERNS:
+            assert dg.pattern_count(p, t)[0] == oracle.pattern(rp, ci, p)[0], \
+                (split, p)

@@ -21,6 +21,8 @@ FORCE = {
     "global_only": dict(small_max_deg=-1, wave_max_deg=-1, block_max_deg=-1),
     "tiny_chunks": dict(chunk_work=7),
     "items4": dict(tc_items=4),
+    "cursor_always": dict(tc_cursor_min=1),
+    "cursor_never": dict(tc_cursor_min=-1),
     "items8": dict(tc_items=8),
     "items8_global": dict(tc_items=8, small_max_deg=-1, wave_max_deg=-1,
                           block_max_deg=-1),
