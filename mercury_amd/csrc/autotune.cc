@@ -121,12 +121,13 @@ int tune_pattern(mgpm_dgraph *dg, int pattern, int reps, mgpm_tune *best,
                  double *best_ms) {
   mgpm_tune cur;
   std::memset(&cur, 0, sizeof(cur));
+  int rc = pattern_make_resident(dg, nullptr, pattern);
+  if (rc) return rc;
   cur.grid_blocks = dg->pat.grid_cap;
-  cur.pattern_split = dg->pat.split;
+  cur.pattern_split = dg->pat.plan[pattern].split;
   uint64_t want = 0;
   double cur_ms = 0;
-  int rc = time_pattern(dg, pattern, cur, reps, &want, &cur_ms);
-  if (rc) return rc;
+  if ((rc = time_pattern(dg, pattern, cur, reps, &want, &cur_ms))) return rc;
   const std::vector<Knob> knobs = {
       {&mgpm_tune::pattern_split, {-1, 16, 18, 20, 22}},
       {&mgpm_tune::grid_blocks, {dg->cus, 2 * dg->cus, 4 * dg->cus}},
@@ -147,7 +148,8 @@ int tune_pattern(mgpm_dgraph *dg, int pattern, int reps, mgpm_tune *best,
       }
     }
   }
-  if ((rc = pattern_make_resident(dg, nullptr, cur.pattern_split))) return rc;
+  if ((rc = pattern_make_resident(dg, nullptr, pattern, cur.pattern_split)))
+    return rc;
   dg->pat.grid_cap = cur.grid_blocks;
   *best = cur;
   *best_ms = cur_ms;

@@ -26,12 +26,11 @@ uint64_t slab_bytes(int64_t cap, int grid) {
 
 uint64_t query_cost(const mgpm_dgraph *g, int pattern) {
   if (pattern == MGPM_TC) return g->tc.probes + g->tc.m_plus + 1;
-  // The pattern plan's costs bound the pair tests of the cubic patterns;
-  // the quadratic ones (P1, P4) are overestimated and simply start early.
+  // The pattern plan's task costs bound the pattern's pair tests.
+  const PatternPlanResident &pr = g->pat.plan[pattern];
   uint64_t c = 0;
-  for (uint64_t x : g->pat.edge_group_cost) c += x;
-  if (pattern == MGPM_P1)
-    for (uint64_t x : g->pat.vert_group_cost) c += x;
+  for (uint64_t x : pr.edge_group_cost) c += x;
+  for (uint64_t x : pr.vert_group_cost) c += x;
   return c + 1;
 }
 
@@ -82,7 +81,7 @@ int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
       return fail(MGPM_ERR_INVALID, "query graph lives on another device");
     if (p < MGPM_TC || p > MGPM_P8) return fail(MGPM_ERR_PATTERN, "Not support.");
     if (p != MGPM_TC) {
-      const int rc = pattern_make_resident(g, nullptr);
+      const int rc = pattern_make_resident(g, nullptr, p);
       if (rc) return rc;
     }
     queries_[i] = Planned{g, p, query_cost(g, p)};
@@ -102,7 +101,7 @@ int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
     load[l] += queries_[i].cost;
     order_[l].push_back(i);
     if (queries_[i].pattern != MGPM_TC)
-      need_cap[l] = std::max<int64_t>(need_cap[l], queries_[i].g->pat.dev.cap);
+      need_cap[l] = std::max<int64_t>(need_cap[l], queries_[i].g->pat.cap);
   }
   // Workspace sizes; grow-only, checked against the limit first.
   std::vector<int> grid(static_cast<size_t>(L), 0);
@@ -170,7 +169,7 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
         if (tc_device_launch(p, tl, ls))
           return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
       } else {
-        PatternDevicePlan p = qp.g->pat.dev;
+        PatternDevicePlan p = pattern_device_plan(qp.g, qp.pattern);
         p.slab = slab_[l].as<int32_t>();
         p.cap = slab_cap_[l];
         p.queues = queues + 4 * i;

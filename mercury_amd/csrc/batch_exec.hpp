@@ -71,7 +71,7 @@ class BatchExec {
 };
 
 // Estimated device work of one query, in the units the plans count
-// (TC: probes plus DAG arcs; patterns: the pattern plan's task costs).
+// (TC: probes plus DAG arcs; patterns: the pattern's plan task costs).
 uint64_t query_cost(const mgpm_dgraph *g, int pattern);
 
 }  // namespace mgpm

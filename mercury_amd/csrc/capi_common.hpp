@@ -85,20 +85,31 @@ struct CsrResident {
   size_t bytes() const { return rowptr.bytes + colidx.bytes; }
 };
 
-// P1..P8 task plan, built on first use, and the per-wave scratch of the
+// One pattern's task plan on the device, built on first use.
+struct PatternPlanResident {
+  bool ready = false;
+  int split = 0;  // shares above 2^split pair tests (-1: no splits)
+  int n_edge_groups = 0, n_vert_groups = 0;
+  DevBuf edges, edge_groups, verts, vert_groups;
+  std::vector<uint64_t> edge_group_cost, vert_group_cost;  // host copies
+  size_t bytes() const {
+    return edges.bytes + edge_groups.bytes + verts.bytes + vert_groups.bytes;
+  }
+};
+
+// P1..P8 plans (one per pattern) and the per-wave scratch of the
 // single-query path, allocated on its first call.
 struct PatternResident {
-  bool ready = false;
   int grid = 0;      // workgroups the single-query slab holds
   int grid_cap = 0;  // cap on persistent workgroups (0 = none)
-  int split = 20;    // shares above 2^split pair tests (-1: no splits)
-  int32_t max_degree = 0;
-  DevBuf edges, edge_groups, verts, vert_groups, slab;
-  std::vector<uint64_t> edge_group_cost, vert_group_cost;  // host copies
-  PatternDevicePlan dev;
+  int split = 20;    // split exponent for plans built from now on
+  int64_t cap = 64;  // slab ints per list (>= max degree)
+  DevBuf slab;
+  PatternPlanResident plan[9];  // index = pattern (1..8)
   size_t bytes() const {
-    return edges.bytes + edge_groups.bytes + verts.bytes + vert_groups.bytes +
-           slab.bytes;
+    size_t b = slab.bytes;
+    for (const auto &p : plan) b += p.bytes();
+    return b;
   }
 };
 
@@ -107,9 +118,13 @@ TcPlanParams tc_params_from(const mgpm_tune *t);
 int set_device(int device);
 int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s);
-// Builds the pattern plan if missing or if `split` (0 = keep the current
-// setting) differs from the resident plan's.
-int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int split = 0);
+// Builds pattern `pattern`'s plan if missing or if `split` (0 = the
+// graph's current setting) differs from the resident plan's.
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int pattern,
+                          int split = 0);
+// Device view of pattern `pattern`'s plan with the graph's own slab,
+// queues and counters.
+PatternDevicePlan pattern_device_plan(const mgpm_dgraph *dg, int pattern);
 int pattern_make_slab(mgpm_dgraph *dg);
 // Tunes the triangle knobs of the largest not-yet-tuned graph of every
 // size class (log2 of DAG arcs) among `graphs` and applies the winner to

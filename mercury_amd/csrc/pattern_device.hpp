@@ -19,7 +19,6 @@ struct PatternDevicePlan {
   int n_vert_groups = 0;
   int32_t *slab = nullptr;          // per-wave scratch, 8 * cap ints each
   int64_t cap = 0;                  // >= max degree
-  int32_t split = 62;               // shares per 2^split pair tests
   unsigned *queues = nullptr;       // 2 counters (edges, vertices)
   unsigned long long *counters = nullptr;  // c1, c2
 };

@@ -33,6 +33,38 @@ __device__ __forceinline__ bool bs_contains(const int32_t *__restrict__ a,
   return lo < n && a[lo] == key;
 }
 
+// First index i of the sorted list a[0, n) with a[i] > key (STRICT) or
+// a[i] >= key.
+template <bool STRICT>
+__device__ __forceinline__ int32_t bound(const int32_t *__restrict__ a,
+                                         int32_t n, int32_t key) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (STRICT ? a[mid] <= key : a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// floor(log2(n)) + 1 for n >= 1: the steps of a binary search over n.
+__device__ __forceinline__ uint32_t search_steps(int32_t n) {
+  return 32u - __clz(n | 1);
+}
+
+// Whether counting |A ∩ B| should walk B (each element searched in A)
+// rather than A (each searched in B), A being a frontier list in the
+// per-wave slab and B an adjacency list.  Each way costs ceil(n/64) wave
+// steps of search_steps(other side) dependent loads; a search in the slab
+// is charged 4 times one in the adjacency lists (the slab is far larger
+// than the caches, a small graph's CSR is not), plus `extra` steps.
+__device__ __forceinline__ bool walk_other_side(int32_t na, int32_t nb,
+                                                uint32_t extra = 0) {
+  const uint32_t walk_a = uint32_t((na + 63) >> 6) * search_steps(nb);
+  const uint32_t walk_b = uint32_t((nb + 63) >> 6) * search_steps(na) + extra;
+  return 4 * walk_b < walk_a;
+}
+
 // out = {x in A : x < ub, (x in B) == KEEP_IN}, order preserved.  Returns
 // the count (wave-uniform).  `out` is wave-private global scratch.
 template <bool KEEP_IN>
@@ -102,12 +134,34 @@ __device__ __forceinline__ uint64_t pair_count(const PGraph &g,
     const int32_t o = O[a];
     const int32_t *No = g.nb(o);
     const int32_t dO = g.deg(o);
-    for (int32_t b = lane; b < ni; b += 64) {
-      const int32_t in = I[b];
-      const int32_t x = outer_x ? o : in, y = outer_x ? in : o;
-      if (MODE != kNonAdj && !(y > x)) continue;
-      const bool adj = bs_contains(No, dO, in);
-      c += (MODE == kAdjYGt) ? adj : !adj;
+    if (!walk_other_side(ni, dO, MODE == kNonAdjYGt ? search_steps(ni) : 0)) {
+      for (int32_t b = lane; b < ni; b += 64) {
+        const int32_t in = I[b];
+        const int32_t x = outer_x ? o : in, y = outer_x ? in : o;
+        if (MODE != kNonAdj && !(y > x)) continue;
+        const bool adj = bs_contains(No, dO, in);
+        c += (MODE == kAdjYGt) ? adj : !adj;
+      }
+      continue;
+    }
+    // Walk N(o) instead: adjacent partners are z in N(o) ∩ I (above o when
+    // o is x, below o when o is y); non-adjacent ones are the eligible
+    // inner elements minus those.
+    uint32_t adj = 0;
+    for (int32_t b = lane; b < dO; b += 64) {
+      const int32_t z = No[b];
+      const bool eligible = MODE == kNonAdj || (outer_x ? z > o : z < o);
+      adj += eligible && bs_contains(I, ni, z);
+    }
+    if (MODE == kAdjYGt) {
+      c += adj;
+    } else {
+      int32_t total = ni;
+      if (MODE != kNonAdj)
+        total = outer_x ? ni - bound<true>(I, ni, o) : bound<false>(I, ni, o);
+      // total - adj, split over the lanes; the wave sum is exact mod 2^64.
+      if (lane == 0) c += static_cast<uint64_t>(total);
+      c -= adj;
     }
   }
   return c;
@@ -157,24 +211,30 @@ __device__ __forceinline__ void split_by(const PGraph &g, int32_t v,
   }
 }
 
-// Number of x in L[0, upto) with (x ~ v) == ADJ, per-lane partial.
+// Number of x in the sorted list L[0, upto) with (x ~ v) == ADJ, as a
+// per-lane partial (the lanes' values may be negative; their sum is exact).
 template <bool ADJ>
-__device__ __forceinline__ uint32_t count_prefix(const PGraph &g, int32_t v,
-                                                 const int32_t *L,
-                                                 int32_t upto) {
+__device__ __forceinline__ int32_t count_prefix(const PGraph &g, int32_t v,
+                                                const int32_t *L,
+                                                int32_t upto) {
   const int32_t *Nv = g.nb(v);
   const int32_t dv = g.deg(v);
-  uint32_t c = 0;
-  for (int32_t i = lane64(); i < upto; i += 64)
-    c += bs_contains(Nv, dv, L[i]) == ADJ;
-  return c;
+  const int lane = lane64();
+  int32_t c = 0;
+  if (!walk_other_side(upto, dv)) {
+    for (int32_t i = lane; i < upto; i += 64)
+      c += bs_contains(Nv, dv, L[i]) == ADJ;
+    return c;
+  }
+  for (int32_t i = lane; i < dv; i += 64) c += bs_contains(L, upto, Nv[i]);
+  if (ADJ) return c;
+  return (lane == 0 ? upto : 0) - c;
 }
 
 // Number of x in L[0, n) not adjacent to v (wave-uniform total).
 __device__ __forceinline__ uint32_t count_nonadj(const PGraph &g, int32_t v,
                                                  const int32_t *L, int32_t n) {
-  return static_cast<uint32_t>(
-      wave_sum_i32(static_cast<int>(count_prefix<false>(g, v, L, n))));
+  return static_cast<uint32_t>(wave_sum_i32(count_prefix<false>(g, v, L, n)));
 }
 
 // ------------------------------------------------------------ P1 vertex
@@ -192,17 +252,10 @@ __device__ void p1_vertex(const PGraph &g, int32_t v0, int32_t sub,
   if (nsub > 1) {
     for (int32_t k = sub; k < d0; k += nsub) {
       const int32_t y = N0[k];
-      const int32_t *Ny = g.nb(y);
-      const int32_t dy = g.deg(y);
-      int before = 0, after = 0;
-      for (int32_t i = lane; i < d0; i += 64) {
-        const bool non = i != k && !bs_contains(Ny, dy, N0[i]);
-        before += non && i < k;
-        after += non && i > k;
-      }
-      before = wave_sum_i32(before);
-      after = wave_sum_i32(after);
-      if (lane == 0) c1 += static_cast<uint64_t>(before) * uint32_t(after);
+      // N0[k] itself is not in N(y) (no self-loops), so it splits cleanly.
+      const uint32_t before = count_nonadj(g, y, N0, k);
+      const uint32_t after = count_nonadj(g, y, N0 + k + 1, d0 - k - 1);
+      if (lane == 0) c1 += static_cast<uint64_t>(before) * after;
     }
     return;
   }
@@ -228,38 +281,18 @@ __device__ void p1_vertex(const PGraph &g, int32_t v0, int32_t sub,
 
 // -------------------------------------------------------------- edges
 
-// Shares a quadratic task (P1's correction, P4: at most d0 * d1 pair
-// tests) keeps of the `nsub` the plan cut for the cubic patterns: one per
-// 2^split pair tests.
-__device__ __forceinline__ int32_t quadratic_shares(int32_t d0, int32_t d1,
-                                                    int32_t split,
-                                                    int32_t nsub) {
-  const uint64_t work = uint64_t(d0) * uint64_t(d1);
-  const uint64_t want = (work >> split) + 1;
-  return want < uint64_t(nsub) ? static_cast<int32_t>(want) : nsub;
-}
-
 // One share of the task edge (v0, v1): outer loops take the indices
 // sub, sub + nsub, ...; nsub == 1 is the whole edge.  Per-index frequency
 // factors that the reference accumulates over the outer loop are computed
-// per index here, so shares of one edge need no common state.  The plan's
-// split exponent `split` trims the share count of the quadratic patterns
-// (shares past it do nothing).
+// per index here, so shares of one edge need no common state.
 template <int P>
 __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
-                          int32_t sub, int32_t nsub, int32_t split, Slab &s,
-                          uint64_t &c1, uint64_t &c2) {
+                          int32_t sub, int32_t nsub, Slab &s, uint64_t &c1,
+                          uint64_t &c2) {
   const int lane = lane64();
   const int32_t *N0 = g.nb(v0), *N1 = g.nb(v1);
   const int32_t d0 = g.deg(v0), d1 = g.deg(v1);
   int32_t **L = s.L;
-  if constexpr (P == 1 || P == 4) {
-    if (nsub > 1) {
-      const int32_t nq = quadratic_shares(d0, d1, split, nsub);
-      if (sub >= nq) return;
-      nsub = nq;
-    }
-  }
   if constexpr (P == 1) {  // P1_count_correction :6-66
     if (v0 == v1) return;
     const int32_t na = wave_diff(N0, d0, N1, d1, v1, L[0]);
@@ -287,9 +320,9 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
       const int32_t y = L[0][i];
       const uint32_t f = count_nonadj(g, y, L[1], n1);
       if (f == 0) continue;
-      const uint32_t part = P == 2 ? count_prefix<false>(g, y, L[0], i)
-                                   : count_prefix<true>(g, y, L[0], i);
-      c1 += static_cast<uint64_t>(part) * f;
+      const int32_t part = P == 2 ? count_prefix<false>(g, y, L[0], i)
+                                  : count_prefix<true>(g, y, L[0], i);
+      c1 += static_cast<uint64_t>(static_cast<int64_t>(part) * f);
     }
   } else if constexpr (P == 3 || P == 7) {  // :245-350, :352-459
     const int32_t n0 = P == 3 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
@@ -341,8 +374,8 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
       const int32_t x = L[1][i];
       const uint32_t f = count_nonadj(g, x, L[0], n0);
       if (f == 0) continue;
-      const uint32_t t = count_prefix<false>(g, x, L[2], n2);
-      c1 += static_cast<uint64_t>(t) * f;
+      const int32_t t = count_prefix<false>(g, x, L[2], n2);
+      c1 += static_cast<uint64_t>(static_cast<int64_t>(t) * f);
     }
   } else if constexpr (P == 6) {  // :736-853
     const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);

@@ -42,7 +42,7 @@ template <int P>
 __global__ __launch_bounds__(kPWaves * 64) void pattern_edges(
     PGraph g, const int4 *__restrict__ edges,
     const int32_t *__restrict__ goff, int ngroups, int32_t *slab,
-    int64_t cap, int split, unsigned *queue, unsigned long long *counters) {
+    int64_t cap, unsigned *queue, unsigned long long *counters) {
   const int lane = lane64();
   const int64_t wave = int64_t(blockIdx.x) * kPWaves + (threadIdx.x >> 6);
   Slab s(slab + wave * 8 * cap, cap);
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kPWaves * 64) void pattern_edges(
     const int t_end = goff[q + 1];
     for (int t = goff[q]; t < t_end; ++t) {
       const int4 e = edges[t];
-      edge_task<P>(g, e.x, e.y, e.z, e.w, split, s, c1, c2);
+      edge_task<P>(g, e.x, e.y, e.z, e.w, s, c1, c2);
     }
   }
   flush2(c1, c2, counters);
@@ -95,7 +95,7 @@ void launch_edges(const PatternDevicePlan &p, int grid, hipStream_t s) {
   PGraph g{p.rowptr, p.col};
   hipLaunchKernelGGL(pattern_edges<P>, dim3(grid), dim3(kPWaves * 64), 0, s,
                      g, p.edges, p.edge_groups, p.n_edge_groups, p.slab,
-                     p.cap, p.split, p.queues, p.counters);
+                     p.cap, p.queues, p.counters);
 }
 
 // Workgroups that can take a queue entry: one entry per wave.

@@ -71,29 +71,64 @@ int32_t shares(uint64_t work, int64_t outer, uint64_t split_work) {
       std::max<int64_t>(1, std::min<int64_t>(static_cast<int64_t>(want), cap)));
 }
 
+// Pair tests of pattern `p` on the task edge (v0, v1) with degrees d0, d1,
+// bounded from the set algebra of pattern_kernels.cuh (L0/L1 are subsets
+// of N(v0), N(v1) or their intersection, m = min(d0, d1)), and the length
+// of the longest outer loop a share strides over.
+struct EdgeCost {
+  uint64_t work;
+  int64_t outer;
+};
+
+EdgeCost edge_cost(int p, uint64_t d0, uint64_t d1) {
+  const uint64_t m = std::min(d0, d1);
+  switch (p) {
+    case 1:  // pairs of I<(N0,N1) x D<(N0,N1); outer = the shorter side
+      return {d0 * m + d0 + d1, static_cast<int64_t>(m)};
+    case 2:  // for z in D(N1,N0): split D(N0,N1), pairs of its two parts;
+             // then for y in D(N0,N1): a prefix count
+      return {d1 * (d0 * d0 / 4 + d0) + d0 * d0 + d1,
+              static_cast<int64_t>(std::max(d0, d1))};
+    case 3:  // for y in D(N0,N1): scan it, pairs of its two parts
+      return {d0 * (d0 * d0 / 4 + d0) + d1, static_cast<int64_t>(d0)};
+    case 4:  // pairs of D(N0,N1) x D(N1,N0)
+      return {d0 * d1 + d0 + d1, static_cast<int64_t>(m)};
+    case 5:  // for c in I: split both differences, pairs across them; then
+             // for a in D(N0,N1): two counts
+      return {m * (d0 * d1 + d0 + d1) + d0 * (m + d1),
+              static_cast<int64_t>(d0)};
+    case 6:  // for c in I: split I and D(N1,N0), pairs across them
+      return {m * (m * d1 + m + d1) + d0 + d1, static_cast<int64_t>(m)};
+    case 7:  // P3's machinery on I(N0,N1)
+      return {m * (m * m / 4 + m) + d0 + d1, static_cast<int64_t>(m)};
+    default:  // 8: for z in D(N1,N0): split I, pairs of its parts; then
+              // for y in I: two counts
+      return {d1 * (m * m / 4 + m) + m * (d1 + m) + d0,
+              static_cast<int64_t>(std::max(d1, m))};
+  }
+}
+
 }  // namespace
 
-void build_pattern_plan(const CsrGraph &g, PatternPlan &plan,
+void build_pattern_plan(const CsrGraph &g, int pattern, PatternPlan &plan,
                         uint64_t split_work) {
   if (split_work == 0) split_work = kPatternSplitWork;
   const int32_t n = g.n;
   plan.max_degree = g.max_degree();
-  // Per directed edge: the pair tests of the heaviest patterns, bounded by
-  // d0 * d1 * min(d0, d1) / 4, and the outer-loop length min(d0, d1).  That
-  // bound (per share) also orders and groups the tasks.
-  auto edge_work = [&](int32_t v, int32_t u) {
-    const uint64_t d0 = g.degree(v) + 1, d1 = g.degree(u) + 1;
-    return d0 * d1 * std::min(d0, d1) / 4 + 1;
-  };
-  std::vector<int32_t> esplit(static_cast<size_t>(g.nnz));
+  // P4 counts each undirected edge once, from its larger endpoint
+  // (pattern_kernels.cuh:878-882 skips v1 >= v0); the others take all 2m
+  // directed edges (graph.cc:336-365).
+  auto keep = [&](int32_t v, int32_t u) { return pattern != 4 || u < v; };
+  std::vector<int32_t> esplit(static_cast<size_t>(g.nnz), 0);
   std::vector<int64_t> eoff(static_cast<size_t>(n) + 1, 0);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int32_t v = 0; v < n; ++v) {
     int64_t acc = 0;
     for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e) {
       const int32_t u = g.colidx[e];
-      const int64_t lo = std::min(g.degree(v), g.degree(u));
-      esplit[e] = shares(edge_work(v, u), lo, split_work);
+      if (!keep(v, u)) continue;
+      const EdgeCost c = edge_cost(pattern, g.degree(v), g.degree(u));
+      esplit[e] = shares(c.work + 1, c.outer, split_work);
       acc += esplit[e];
     }
     eoff[v + 1] = acc;
@@ -106,8 +141,10 @@ void build_pattern_plan(const CsrGraph &g, PatternPlan &plan,
     int64_t k = eoff[v];
     for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e) {
       const int32_t u = g.colidx[e];
+      if (!keep(v, u)) continue;
+      const EdgeCost c = edge_cost(pattern, g.degree(v), g.degree(u));
       // Shares of a split edge each get a queue entry of their own.
-      const uint64_t w = edge_work(v, u) / uint64_t(esplit[e]) + 1;
+      const uint64_t w = c.work / uint64_t(esplit[e]) + 1;
       const uint64_t share = esplit[e] > 1 ? std::max(w, kGroupCost) : w;
       for (int32_t s = 0; s < esplit[e]; ++s, ++k) {
         tasks[k] = PatternTask{v, u, s, esplit[e]};
@@ -118,6 +155,10 @@ void build_pattern_plan(const CsrGraph &g, PatternPlan &plan,
   order_and_group(
       static_cast<int64_t>(tasks.size()), [&](int64_t i) { return tcost[i]; },
       tasks, plan.edges, plan.edge_groups, plan.edge_group_cost);
+  plan.verts.clear();
+  plan.vert_groups.assign(1, 0);
+  plan.vert_group_cost.clear();
+  if (pattern != 1) return;
   // P1 vertices: d(d-1)/2 pair tests, outer loop over d.
   std::vector<PatternTask> vt;
   std::vector<uint64_t> vcost;

@@ -1,8 +1,9 @@
-// Host-side task plan of the P1..P8 kernels: every directed edge (the
-// reference's COO task list, graph.cc:336-365) and, for P1, every vertex,
-// heavy ones split into shares (a share takes every nsub-th index of the
-// task's outer loops), ordered by an estimate of their set-algebra cost and
-// packed into queue groups (heavy tasks alone, light ones together).
+// Host-side task plan of one of the P1..P8 kernels: every directed edge
+// (the reference's COO task list, graph.cc:336-365; P4 only those with
+// v1 < v0) and, for P1, every vertex, heavy ones split into shares (a share
+// takes every nsub-th index of the task's outer loops), ordered by the
+// pattern's own bound on its pair tests and packed into queue groups
+// (heavy tasks alone, light ones together).
 #pragma once
 
 #include <cstdint>
@@ -33,8 +34,9 @@ struct PatternPlan {
   int32_t max_degree = 0;
 };
 
-// split_work = 0 takes kPatternSplitWork; a huge value disables splits.
-void build_pattern_plan(const CsrGraph &g, PatternPlan &plan,
+// Plan of pattern `pattern` (1..8).  split_work = 0 takes
+// kPatternSplitWork; a huge value disables splits.
+void build_pattern_plan(const CsrGraph &g, int pattern, PatternPlan &plan,
                         uint64_t split_work = 0);
 
 }  // namespace mgpm

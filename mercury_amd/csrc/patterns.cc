@@ -25,43 +25,51 @@ int upload(DevBuf &buf, const std::vector<T> &v, hipStream_t s) {
 
 }  // namespace
 
-int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int split) {
+int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int pattern,
+                          int split) {
   PatternResident &r = dg->pat;
-  if (split == 0) split = r.split;
-  if (r.ready && split == r.split) return MGPM_OK;
-  const bool replace = r.ready;
+  PatternPlanResident &pr = r.plan[pattern];
+  if (split == 0) split = pr.ready ? pr.split : r.split;
+  if (pr.ready && split == pr.split) return MGPM_OK;
+  const bool replace = pr.ready;
   PatternPlan plan;
-  build_pattern_plan(*dg->host, plan,
+  build_pattern_plan(*dg->host, pattern, plan,
                      split < 0 ? UINT64_MAX : uint64_t(1) << std::min(split, 62));
   int rc;
-  if ((rc = upload(r.edges, plan.edges, s))) return rc;
-  if ((rc = upload(r.edge_groups, plan.edge_groups, s))) return rc;
-  if ((rc = upload(r.verts, plan.verts, s))) return rc;
-  if ((rc = upload(r.vert_groups, plan.vert_groups, s))) return rc;
-  const int64_t cap = std::max<int64_t>(plan.max_degree, 64);
+  if ((rc = upload(pr.edges, plan.edges, s))) return rc;
+  if ((rc = upload(pr.edge_groups, plan.edge_groups, s))) return rc;
+  if ((rc = upload(pr.verts, plan.verts, s))) return rc;
+  if ((rc = upload(pr.vert_groups, plan.vert_groups, s))) return rc;
   MGPM_HIP(hipStreamSynchronize(s));
-  PatternDevicePlan &d = r.dev;
+  pr.n_edge_groups = static_cast<int>(plan.edge_groups.size()) - 1;
+  pr.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
+  pr.edge_group_cost = std::move(plan.edge_group_cost);
+  pr.vert_group_cost = std::move(plan.vert_group_cost);
+  pr.split = split;
+  pr.ready = true;
+  r.cap = std::max<int64_t>(plan.max_degree, 64);
+  if (replace) dg->tc.generation++;  // batches holding the old plan re-plan
+  return MGPM_OK;
+}
+
+PatternDevicePlan pattern_device_plan(const mgpm_dgraph *dg, int pattern) {
+  const PatternResident &r = dg->pat;
+  const PatternPlanResident &pr = r.plan[pattern];
+  PatternDevicePlan d;
   d.rowptr = dg->csr.rowptr.as<int64_t>();
   d.col = dg->csr.colidx.as<int32_t>();
-  d.edges = r.edges.as<int4>();
-  d.edge_groups = r.edge_groups.as<int32_t>();
-  d.n_edge_groups = static_cast<int>(plan.edge_groups.size()) - 1;
-  d.verts = r.verts.as<int4>();
-  d.vert_groups = r.vert_groups.as<int32_t>();
-  d.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
-  d.slab = r.slab.as<int32_t>();  // kept across re-plans (same cap)
-  d.cap = cap;
-  d.split = split < 0 ? 62 : std::min(split, 62);
+  d.edges = pr.edges.as<int4>();
+  d.edge_groups = pr.edge_groups.as<int32_t>();
+  d.n_edge_groups = pr.n_edge_groups;
+  d.verts = pr.verts.as<int4>();
+  d.vert_groups = pr.vert_groups.as<int32_t>();
+  d.n_vert_groups = pr.n_vert_groups;
+  d.slab = r.slab.as<int32_t>();
+  d.cap = r.cap;
   d.queues = reinterpret_cast<unsigned *>(dg->scratch.as<char>() + 128);
   d.counters =
       reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 192);
-  r.split = split;
-  r.max_degree = plan.max_degree;
-  if (replace) dg->tc.generation++;  // batches holding the old plan re-plan
-  r.edge_group_cost = std::move(plan.edge_group_cost);
-  r.vert_group_cost = std::move(plan.vert_group_cost);
-  r.ready = true;
-  return MGPM_OK;
+  return d;
 }
 
 int pattern_make_slab(mgpm_dgraph *dg) {
@@ -69,11 +77,10 @@ int pattern_make_slab(mgpm_dgraph *dg) {
   if (r.slab.p) return MGPM_OK;
   const int wpb = pattern_waves_per_block();
   int grid = dg->cus * 4;  // 4 workgroups of 4 waves per CU
-  const uint64_t per_block = uint64_t(wpb) * 8 * r.dev.cap * sizeof(int32_t);
+  const uint64_t per_block = uint64_t(wpb) * 8 * r.cap * sizeof(int32_t);
   grid = static_cast<int>(
       std::max<uint64_t>(1, std::min<uint64_t>(grid, kSlabBudget / per_block)));
   MGPM_HIP(r.slab.alloc(per_block * grid));
-  r.dev.slab = r.slab.as<int32_t>();
   r.grid = grid;
   return MGPM_OK;
 }
@@ -96,10 +103,10 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
     hipStream_t s = static_cast<hipStream_t>(stream);
     double prep = 0;
     const int split = tune ? tune->pattern_split : 0;
-    if (!dg->pat.ready || !dg->pat.slab.p ||
-        (split != 0 && split != dg->pat.split)) {
+    const PatternPlanResident &pr = dg->pat.plan[pattern];
+    if (!pr.ready || !dg->pat.slab.p || (split != 0 && split != pr.split)) {
       const auto tp = std::chrono::steady_clock::now();
-      if ((rc = pattern_make_resident(dg, s, split))) return rc;
+      if ((rc = pattern_make_resident(dg, s, pattern, split))) return rc;
       if ((rc = pattern_make_slab(dg))) return rc;
       prep = ms_since(tp);
     }
@@ -108,10 +115,11 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
                                                     : dg->pat.grid_cap;
     if (cap > 0) grid = std::min(grid, cap);
     MGPM_HIP(hipEventRecord(dg->ev0, s));
-    if (pattern_device_launch(dg->pat.dev, pattern, grid, s))
+    if (pattern_device_launch(pattern_device_plan(dg, pattern), pattern, grid,
+                              s))
       return fail(MGPM_ERR_HIP, "pattern kernel launch failed");
     MGPM_HIP(hipEventRecord(dg->ev1, s));
-    MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->pat.dev.counters, 16,
+    MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->scratch.as<char>() + 192, 16,
                             hipMemcpyDeviceToHost, s));
     MGPM_HIP(hipStreamSynchronize(s));
     float kms = 0;

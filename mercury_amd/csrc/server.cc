@@ -189,13 +189,15 @@ int mgpm_server_add_graph(mgpm_server *s, const mgpm_graph *g,
   try {
     int rc = set_device(s->cfg.device);
     if (rc) return rc;
-    // Upload (TC plan) and build the pattern plan, then admit by footprint
-    // (slots hold the pattern scratch, so the graph holds none).
+    // Upload (TC plan) and build every pattern's plan, then admit by
+    // footprint (slots hold the pattern scratch, so the graph holds none).
     mgpm_dgraph *dg = nullptr;
     if ((rc = mgpm_dgraph_upload(g, s->cfg.device, nullptr, &dg))) return rc;
-    if ((rc = pattern_make_resident(dg, nullptr))) {
-      mgpm_dgraph_free(dg);
-      return rc;
+    for (int p = MGPM_P1; p <= MGPM_P8; ++p) {
+      if ((rc = pattern_make_resident(dg, nullptr, p))) {
+        mgpm_dgraph_free(dg);
+        return rc;
+      }
     }
     uint64_t bytes = 0;
     mgpm_dgraph_footprint(dg, &bytes);

@@ -24,10 +24,10 @@ __device__ __forceinline__ uint32_t stream_partners(
     const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
     int first_batch, int batch_stride, int32_t *mark_lds,
     const Probe &probe, int cursor_min) {
-  // The start marks are written by one lane and read by another; a wave's
-  // LDS instructions execute in order, so `volatile` (no compiler
-  // reordering or store forwarding) is all the ordering needed.
-  volatile int32_t *mark = mark_lds;
+  // The start marks are written by one lane and read by another.  They are
+  // LDS accesses (ds_*), which a wave executes in order; lds_order() keeps
+  // the compiler from reordering or forwarding them across each handoff.
+  lds_i32 *mark = as_lds(mark_lds);
   const int lane = lane64();
   uint32_t hits = 0;
   for (int pb = first_batch * 64; pb < pcnt; pb += batch_stride * 64) {
@@ -48,8 +48,10 @@ __device__ __forceinline__ uint32_t stream_partners(
       // divergent code, where cross-lane shuffles would read inactive
       // lanes).  Items lane + 64 i advance the cursor by at most a few
       // partners per step.
+      lds_order();  // previous reads of the marks are done
       mark[lane] = incl;
       mark[64 + lane] = static_cast<int32_t>(beg);
+      lds_order();
       int o = 0, oexcl = 0, onext = mark[0];
       uint32_t obeg = static_cast<uint32_t>(mark[64]);
       for (int k = 0; k < total; k += 64 * ITEMS) {
@@ -76,15 +78,20 @@ __device__ __forceinline__ uint32_t stream_partners(
       continue;
     }
     for (int k = 0; k < total; k += (64 * ITEMS)) {
+      lds_order();  // previous reads of the marks are done
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;
+      lds_order();
       if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
+      lds_order();
       unsigned long long starts[ITEMS];
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i)
         starts[i] = __ballot(mark[lane + 64 * i] != 0);
-      // The partner owning item g is the last one with excl <= g.
+      // The partner owning item g is the last one with excl <= g; item g
+      // of the batch is col[g + (beg - excl)] of its owner.
       int base = __popcll(__ballot(excl <= k)) - 1;
+      const int delta = static_cast<int>(beg - static_cast<uint32_t>(excl));
       int32_t w[ITEMS];
       bool ok[ITEMS];
 #pragma unroll
@@ -92,11 +99,10 @@ __device__ __forceinline__ uint32_t stream_partners(
         const int owner = base + bits_below(starts[i]) +
                           static_cast<int>((starts[i] >> lane) & 1ull);
         base += __popcll(starts[i]);
-        const uint32_t obeg =
-            static_cast<uint32_t>(__shfl(static_cast<int>(beg), owner, 64));
-        const int oexcl = __shfl(excl, owner, 64);
+        const uint32_t odelta =
+            static_cast<uint32_t>(__shfl(delta, owner, 64));
         const int item = k + 64 * i + lane;
-        const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+        const uint32_t idx = static_cast<uint32_t>(item) + odelta;
         ok[i] = item < total && idx < ncol;
         w[i] = ok[i] ? col[idx] : -2;
       }

@@ -20,6 +20,20 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS-typed int pointer: accesses through it are ds_read/ds_write.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
+__device__ __forceinline__ lds_i32 *as_lds(int32_t *p) {
+  return (lds_i32 *)p;
+}
+
+// Compiler-only ordering point for LDS handoffs between the lanes of one
+// wave: a wave's DS instructions execute and complete in program order, so
+// keeping the compiler from reordering, merging or forwarding them across
+// this point is all that is needed (no s_waitcnt, unlike a fence or a
+// volatile access).
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
 // Number of set bits of `m` strictly below the calling lane.
 __device__ __forceinline__ int bits_below(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
