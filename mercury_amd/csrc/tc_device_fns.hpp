@@ -104,7 +104,9 @@ __device__ __forceinline__ uint32_t stream_partners(
         const int item = k + 64 * i + lane;
         const uint32_t idx = static_cast<uint32_t>(item) + odelta;
         ok[i] = item < total && idx < ncol;
-        w[i] = ok[i] ? col[idx] : -2;
+        // Unconditional gather (idle items read col[0]): no branch per item.
+        const int32_t v = col[ok[i] ? idx : 0u];
+        w[i] = ok[i] ? v : -2;
       }
       hits += probe.template lookup<ITEMS>(w, ok);
     }
@@ -112,7 +114,16 @@ __device__ __forceinline__ uint32_t stream_partners(
   return hits;
 }
 
-// Open-addressing hash set in LDS, linear probing, empty = -1.
+// Open-addressing hash set in LDS, linear probing, empty = -1.  Keys are
+// vertex ids (>= 0), so an idle item (w = -2) never matches and its probe
+// ends at the first empty slot (tables are at most half full).
+//
+// JOINT: all items of a lane probe together -- each round advances every
+// item still probing and issues all their reads at once (finished items
+// re-read their own slot), so a round costs one LDS round trip however
+// many items miss.  Otherwise each item runs its own probe loop, which is
+// cheaper when tables are tiny and a second probe is rare.
+template <bool JOINT>
 struct HashProbe {
   const int32_t *table;
   int logp;
@@ -128,14 +139,32 @@ struct HashProbe {
       s[i] = table[h[i]];
     }
     uint32_t hits = 0;
+    if constexpr (JOINT) {
+      for (;;) {
+        bool more = false;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      if (!ok[i]) continue;
-      while (s[i] != w[i] && s[i] != -1) {
-        h[i] = (h[i] + 1) & mask;
-        s[i] = table[h[i]];
+        for (int i = 0; i < ITEMS; ++i) {
+          const bool miss = s[i] != w[i] && s[i] != -1;
+          h[i] = miss ? ((h[i] + 1) & mask) : h[i];
+          more |= miss;
+        }
+        if (!more) break;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) s[i] = table[h[i]];
       }
-      hits += s[i] == w[i] ? 1u : 0u;
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        hits += (ok[i] && s[i] == w[i]) ? 1u : 0u;
+    } else {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (!ok[i]) continue;
+        while (s[i] != w[i] && s[i] != -1) {
+          h[i] = (h[i] + 1) & mask;
+          s[i] = table[h[i]];
+        }
+        hits += s[i] == w[i] ? 1u : 0u;
+      }
     }
     return hits;
   }
@@ -208,7 +237,16 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
   wave_sync();
   table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
   wave_sync();
-  HashProbe probe{table, logp, static_cast<uint32_t>(P - 1)};
+  // Tables of up to 256 slots (small owners) probe item by item.
+  if (logp_max <= 8) {
+    HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
+    const uint32_t hits = stream_partners<ITEMS>(
+        col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe,
+        cursor_min);
+    wave_sync();
+    return hits;
+  }
+  HashProbe<true> probe{table, logp, static_cast<uint32_t>(P - 1)};
   const uint32_t hits = stream_partners<ITEMS>(
       col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min);
   wave_sync();

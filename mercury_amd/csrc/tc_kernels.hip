@@ -93,7 +93,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     table_insert(s_table, logp, col, task.tbeg, task.tlen, threadIdx.x,
                  WAVES * 64);
     __syncthreads();
-    HashProbe probe{s_table, logp, static_cast<uint32_t>(P - 1)};
+    HashProbe<true> probe{s_table, logp, static_cast<uint32_t>(P - 1)};
     acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
                            WAVES, s_mark[wv], probe, cursor_min);
     __syncthreads();
