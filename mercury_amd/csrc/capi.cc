@@ -80,7 +80,9 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s) {
   TcPlan plan;
   int rc = build_tc_plan(*dg->host, params, plan);
-  if (rc) return fail(rc, "triangle plan: DAG has 2^32 or more arcs");
+  if (rc == MGPM_ERR_CAPACITY)
+    return fail(rc, "triangle plan: DAG has 2^32 or more arcs");
+  if (rc) return fail(rc, "triangle plan: a list lies outside the DAG");
   TcResident &r = dg->tc;
   r.params = params;
   r.generation++;

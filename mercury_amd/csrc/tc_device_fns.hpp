@@ -85,28 +85,45 @@ __device__ __forceinline__ uint32_t stream_partners(
       if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
       lds_order();
       unsigned long long starts[ITEMS];
+      int own[ITEMS];  // a partner starts at this lane's item i
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i)
-        starts[i] = __ballot(mark[lane + 64 * i] != 0);
+      for (int i = 0; i < ITEMS; ++i) {
+        const bool st = mark[lane + 64 * i] != 0;
+        starts[i] = __ballot(st);
+        own[i] = st ? 1 : 0;
+      }
       // The partner owning item g is the last one with excl <= g; item g
-      // of the batch is col[g + (beg - excl)] of its owner.
+      // of the batch is col[g + (beg - excl)] of its owner.  Every partner
+      // list lies inside the DAG column array (checked when the plan is
+      // built), so only items past the batch's end need a guard.
       int base = __popcll(__ballot(excl <= k)) - 1;
       const int delta = static_cast<int>(beg - static_cast<uint32_t>(excl));
       int32_t w[ITEMS];
       bool ok[ITEMS];
+      if (k + 64 * ITEMS <= total) {  // a full step: no guards
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        const int owner = base + bits_below(starts[i]) +
-                          static_cast<int>((starts[i] >> lane) & 1ull);
-        base += __popcll(starts[i]);
-        const uint32_t odelta =
-            static_cast<uint32_t>(__shfl(delta, owner, 64));
-        const int item = k + 64 * i + lane;
-        const uint32_t idx = static_cast<uint32_t>(item) + odelta;
-        ok[i] = item < total && idx < ncol;
-        // Unconditional gather (idle items read col[0]): no branch per item.
-        const int32_t v = col[ok[i] ? idx : 0u];
-        w[i] = ok[i] ? v : -2;
+        for (int i = 0; i < ITEMS; ++i) {
+          const int owner = base + bits_below(starts[i]) + own[i];
+          base += __popcll(starts[i]);
+          const uint32_t odelta = static_cast<uint32_t>(
+              __builtin_amdgcn_ds_bpermute(owner << 2, delta));
+          w[i] = col[static_cast<uint32_t>(k + 64 * i + lane) + odelta];
+          ok[i] = true;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+          const int owner = base + bits_below(starts[i]) + own[i];
+          base += __popcll(starts[i]);
+          const uint32_t odelta = static_cast<uint32_t>(
+              __builtin_amdgcn_ds_bpermute(owner << 2, delta));
+          const int item = k + 64 * i + lane;
+          ok[i] = item < total;
+          // Unconditional gather (idle items read col[0]): no branch.
+          const int32_t v =
+              col[ok[i] ? static_cast<uint32_t>(item) + odelta : 0u];
+          w[i] = ok[i] ? v : -2;
+        }
       }
       hits += probe.template lookup<ITEMS>(w, ok);
     }
@@ -230,7 +247,8 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
     return stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0,
                                   1, mark, probe, cursor_min);
   }
-  int logp = ceil_log2(4 * task.tlen);
+  // 8 slots per key where the table fits (load <= 1/8): fewer probe rounds.
+  int logp = ceil_log2(8 * task.tlen);
   logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
   const int P = 1 << logp;
   for (int i = lane; i < P; i += 64) table[i] = -1;

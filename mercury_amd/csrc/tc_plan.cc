@@ -136,6 +136,18 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
       plan.group_cost[c].push_back(group_cost[c]);
     }
   plan.dag_col.swap(dag.colidx);
+  // The kernels gather partner and owner lists without bounds checks: every
+  // list must lie inside the DAG column array.
+  const uint64_t ncol = plan.dag_col.size();
+  bool inside = true;
+#pragma omp parallel for reduction(&& : inside)
+  for (int64_t i = 0; i < static_cast<int64_t>(plan.partners.size()); ++i)
+    inside = inside && uint64_t(plan.partners[i].beg) + plan.partners[i].len <= ncol;
+  for (int c = 0; c < kTcNumClasses; ++c)
+    for (const TcTask &t : plan.tasks[c])
+      inside = inside && t.tlen >= 0 && uint64_t(t.tbeg) + uint64_t(t.tlen) <= ncol &&
+               uint64_t(t.pbeg) + uint64_t(t.pcnt) <= plan.partners.size();
+  if (!inside) return MGPM_ERR_INTERNAL;
   return MGPM_OK;
 }
 
