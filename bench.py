@@ -160,7 +160,8 @@ def bench_batch(ma, args) -> int:
                                "uniform in {4,8,16,32}, seed 5; queries "
                                f"{'+'.join(names)} per graph",
                    "queries": nq, "lanes": args.batch_lanes,
-                   "hip_graph": bool(args.batch_use_graph)},
+                   "hip_graph": bool(args.batch_use_graph),
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4")},
         "per_graph_launch_qps": nq / med(t_single),
         "speedup_vs_per_graph_launch": med(t_single) / med(t_batch),
         "counts_equal": counts == single,
@@ -228,7 +229,8 @@ def bench_serve(ma, args) -> int:
                                f"seed 6), Poisson arrivals at "
                                f"{args.serve_rate:g} req/s, {n} requests, "
                                f"pattern uniform in {{{','.join(names)}}}",
-                   "slots": args.serve_slots, "lanes": args.serve_lanes},
+                   "slots": args.serve_slots, "lanes": args.serve_lanes,
+                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4")},
         "p50_ms": served["p50_ms"], "p99_ms": served["p99_ms"],
         "served": served,
         "one_launch_per_request": single,
@@ -260,8 +262,11 @@ def main() -> int:
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-slots", type=int, default=3)
-    ap.add_argument("--serve-lanes", type=int, default=2)
-    ap.add_argument("--batch-lanes", type=int, default=4)
+    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--batch-lanes", type=int, default=8)
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="batch/serve modes: GPU_MAX_HW_QUEUES for this "
+                         "process (0 = leave the environment's value)")
     ap.add_argument("--batch-use-graph", type=int, default=0)
     ap.add_argument("--batch-graphs", type=int, default=512)
     ap.add_argument("--batch-scales", type=int, nargs=2, default=(10, 13),
@@ -280,6 +285,11 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    if args.mode in ("batch", "serve") and args.hw_queues > 0:
+        # Read by the HIP runtime when it starts (before any HIP call): how
+        # many hardware queues this process's streams may spread over, so
+        # how many lanes of a batch can run on the device at once.
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import mercury_amd as ma
     if ma.device_count() < 1:
         print("no GPU visible", file=sys.stderr)

@@ -110,6 +110,7 @@ enum PairMode : int {
   kNonAdj = 0,     // x !~ y
   kNonAdjYGt = 1,  // y > x and x !~ y
   kAdjYGt = 2,     // y > x and x ~ y
+  kAdj = 3,        // x ~ y
 };
 
 // Number of pairs (x in X, y in Y) satisfying MODE; per-lane partial.
@@ -134,13 +135,15 @@ __device__ __forceinline__ uint64_t pair_count(const PGraph &g,
     const int32_t o = O[a];
     const int32_t *No = g.nb(o);
     const int32_t dO = g.deg(o);
+    constexpr bool kOrdered = MODE == kNonAdjYGt || MODE == kAdjYGt;
+    constexpr bool kCountAdj = MODE == kAdjYGt || MODE == kAdj;
     if (!walk_other_side(ni, dO, MODE == kNonAdjYGt ? search_steps(ni) : 0)) {
       for (int32_t b = lane; b < ni; b += 64) {
         const int32_t in = I[b];
         const int32_t x = outer_x ? o : in, y = outer_x ? in : o;
-        if (MODE != kNonAdj && !(y > x)) continue;
+        if (kOrdered && !(y > x)) continue;
         const bool adj = bs_contains(No, dO, in);
-        c += (MODE == kAdjYGt) ? adj : !adj;
+        c += kCountAdj ? adj : !adj;
       }
       continue;
     }
@@ -150,10 +153,10 @@ __device__ __forceinline__ uint64_t pair_count(const PGraph &g,
     uint32_t adj = 0;
     for (int32_t b = lane; b < dO; b += 64) {
       const int32_t z = No[b];
-      const bool eligible = MODE == kNonAdj || (outer_x ? z > o : z < o);
+      const bool eligible = !kOrdered || (outer_x ? z > o : z < o);
       adj += eligible && bs_contains(I, ni, z);
     }
-    if (MODE == kAdjYGt) {
+    if (kCountAdj) {
       c += adj;
     } else {
       int32_t total = ni;
@@ -237,6 +240,29 @@ __device__ __forceinline__ uint32_t count_nonadj(const PGraph &g, int32_t v,
   return static_cast<uint32_t>(wave_sum_i32(count_prefix<false>(g, v, L, n)));
 }
 
+// |N(v) ∩ L[0, n)| for a sorted list L (wave-uniform), and with ABOVE also
+// the part above v in *above.  The shorter of the two lists is walked and
+// each element searched in the other.
+template <bool ABOVE>
+__device__ __forceinline__ uint32_t adj_in(const PGraph &g, int32_t v,
+                                           const int32_t *L, int32_t n,
+                                           uint32_t *above = nullptr) {
+  const int32_t *Nv = g.nb(v);
+  const int32_t dv = g.deg(v);
+  const bool walk_n = dv <= n;
+  const int32_t *W = walk_n ? Nv : L, *S = walk_n ? L : Nv;
+  const int32_t nw = walk_n ? dv : n, ns = walk_n ? n : dv;
+  int c = 0, u = 0;
+  for (int32_t i = lane64(); i < nw; i += 64) {
+    const int32_t x = W[i];
+    const bool in = bs_contains(S, ns, x);
+    c += in;
+    if (ABOVE) u += in && x > v;
+  }
+  if (ABOVE) *above = static_cast<uint32_t>(wave_sum_i32(u));
+  return static_cast<uint32_t>(wave_sum_i32(c));
+}
+
 // ------------------------------------------------------------ P1 vertex
 
 // P1_frequency_count (pattern_kernels.cuh:68-133) for vertex v0:
@@ -300,33 +326,82 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
     wave_sync();
     c2 += pair_count<kNonAdjYGt>(g, L[1], nb, L[0], na, sub, nsub);
   } else if constexpr (P == 2 || P == 8) {  // :135-243, :461-572
+    // The reference's S - C for one task edge is, over z in L1 = D(N1,N0)
+    // with A_z = L0 ∩ N(z), the non-adjacent pairs (P2, L0 = D(N0,N1)) or
+    // the adjacent pairs (P8, L0 = I(N0,N1)) of L0 \ A_z.  Expanding the
+    // edges of L0 \ A_z:  E(L0 \ A_z) = E(L0) - sum_{a in A_z} d(a) + E(A_z)
+    // with d(a) = |N(a) ∩ L0|, and sum_z sum_{a in A_z} d(a) =
+    // sum_a d(a) g(a), g(a) = |N(a) ∩ L1|.  So per a in L0: n1 e+(a) and
+    // d(a) g(a) (e+ = the part of N(a) ∩ L0 above a, counting E(L0) once);
+    // per z in L1: C(|L0| - |A_z|, 2) (P2) and E(A_z).  Every term is an
+    // integer per element, so shares need no common state.
     const int32_t n0 = P == 2 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
                               : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
     wave_sync();
-    for (int32_t i = sub; i < n1; i += nsub) {
-      const int32_t v4 = L[1][i];
-      int32_t n2, n3;
-      // L2 = D(L0, N(v4)), L3 = I(L0, N(v4))
-      split_by(g, v4, L[0], n0, L[2], n2, L[3], n3, nullptr);
-      wave_sync();
-      c2 += P == 2 ? pair_count<kNonAdjYGt>(g, L[3], n3, L[2], n2)
-                   : pair_count<kAdjYGt>(g, L[3], n3, L[2], n2);
-      wave_sync();
-    }
-    // c1 += |D<(L0,N(y),y)| f  (P2)  or  |I<(L0,N(y),y)| f  (P8), with
-    // f = #{v4 in L1 : y !~ v4}
     for (int32_t i = sub; i < n0; i += nsub) {
-      const int32_t y = L[0][i];
-      const uint32_t f = count_nonadj(g, y, L[1], n1);
-      if (f == 0) continue;
-      const int32_t part = P == 2 ? count_prefix<false>(g, y, L[0], i)
-                                  : count_prefix<true>(g, y, L[0], i);
-      c1 += static_cast<uint64_t>(static_cast<int64_t>(part) * f);
+      const int32_t a = L[0][i];
+      uint32_t e = 0;
+      const uint32_t d = adj_in<true>(g, a, L[0], n0, &e);
+      const uint32_t gz = adj_in<false>(g, a, L[1], n1);
+      if (lane == 0) {
+        const uint64_t ne = static_cast<uint64_t>(n1) * e;
+        const uint64_t dg = static_cast<uint64_t>(d) * gz;
+        if (P == 2) { c2 += ne; c1 += dg; }
+        else { c1 += ne; c2 += dg; }
+      }
     }
-  } else if constexpr (P == 3 || P == 7) {  // :245-350, :352-459
-    const int32_t n0 = P == 3 ? wave_diff(N0, d0, N1, d1, kNoUb, L[0])
-                              : wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
+    for (int32_t i = sub; i < n1; i += nsub) {
+      const int32_t z = L[1][i];
+      const int32_t k = wave_inter(g.nb(z), g.deg(z), L[0], n0, kNoUb, L[2]);
+      wave_sync();
+      const uint64_t ea = pair_count<kAdjYGt>(g, L[2], k, L[2], k);
+      if (P == 2) {
+        if (lane == 0) {
+          const uint64_t r = static_cast<uint64_t>(n0 - k);
+          c1 += r * (r - (r > 0 ? 1 : 0)) / 2;
+        }
+        c2 += ea;
+      } else {
+        c1 += ea;
+      }
+      wave_sync();
+    }
+  } else if constexpr (P == 3) {  // :245-350
+    // The reference's S - C for one task edge is the number of independent
+    // 3-subsets of L0 = D(N0,N1).  By
+    // inclusion-exclusion over the three possible edges of a triple:
+    //   indep3 = C(n,3) - (n-2) E + sum_v C(d_v, 2) - T
+    // with d_v = |N(v) ∩ L0|, E the edges inside L0 and T its triangles.
+    // E and T are counted once each by orientation (u > v), so every term
+    // is an integer per vertex v and the shares of one edge need no common
+    // state: c1 takes the added terms, c2 the subtracted ones.
+    const int32_t n0 = wave_diff(N0, d0, N1, d1, kNoUb, L[0]);
+    wave_sync();
+    if (sub == 0 && lane == 0 && n0 >= 3) {
+      const uint64_t n = static_cast<uint64_t>(n0);
+      c1 += n * (n - 1) * (n - 2) / 6;
+    }
+    for (int32_t i = sub; i < n0; i += nsub) {
+      const int32_t v = L[0][i];
+      // M = N(v) ∩ L0 (sorted), and M+ = its part above v.
+      const int32_t dm = wave_inter(g.nb(v), g.deg(v), L[0], n0, kNoUb, L[1]);
+      wave_sync();
+      const int32_t pos = bound<true>(L[1], dm, v);
+      const int32_t up = dm - pos;
+      if (lane == 0) {
+        const uint64_t d = static_cast<uint64_t>(dm);
+        c1 += d * (d - (d > 0 ? 1 : 0)) / 2;                       // C(d_v, 2)
+        c2 += static_cast<uint64_t>(n0 - 2) * static_cast<uint64_t>(up);  // (n-2) e_v+
+      }
+      // Triangles v < a < b inside L0: adjacent pairs within M+.
+      c2 += pair_count<kAdjYGt>(g, L[1] + pos, up, L[1] + pos, up);
+      wave_sync();
+    }
+  } else if constexpr (P == 7) {  // :352-459
+    // Independent 3-subsets of L0 = I(N0,N1) by the reference's triple
+    // machinery (small dense lists: cheaper than the counting of P3).
+    const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     wave_sync();
     for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t y = L[0][i];
@@ -355,40 +430,71 @@ __device__ void edge_task(const PGraph &g, int32_t v0, int32_t v1,
       wave_sync();
     }
   } else if constexpr (P == 5) {  // :574-734
+    // S - C = sum over c in L0 = I(N0,N1) of the non-adjacent pairs of
+    // (L1 \ A_c) x (L2 \ B_c), L1 = D(N0,N1), L2 = D(N1,N0), A_c = L1 ∩ N(c),
+    // B_c = L2 ∩ N(c).  With E(X,Y) the cross edges,
+    //   E(L1\A, L2\B) = E(L1,L2) - E(A,L2) - E(L1,B) + E(A,B),
+    // and summed over c: sum_c E(A_c,L2) = sum_{a in L1} q(a) d2(a),
+    // sum_c E(L1,B_c) = sum_{b in L2} q(b) d1(b) (q = |N(.) ∩ L0|,
+    // d1/d2 = |N(.) ∩ L1/L2|), E(L1,L2) = sum_a d2(a).  Per a in L1:
+    // (q(a) - n0) d2(a); per b in L2: q(b) d1(b); per c in L0:
+    // |L1\A_c| |L2\B_c| - E(A_c,B_c).
     const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N0, d0, N1, d1, kNoUb, L[1]);
     const int32_t n2 = wave_diff(N1, d1, N0, d0, kNoUb, L[2]);
     wave_sync();
-    for (int32_t i = sub; i < n0; i += nsub) {
-      const int32_t v2 = L[0][i];
-      int32_t n4, n5, unused;
-      // L4 = D(L1, N(v2)); L5 = I(L2, N(v2))
-      split_by(g, v2, L[1], n1, L[4], n4, nullptr, unused, nullptr);
-      split_by(g, v2, L[2], n2, nullptr, unused, L[5], n5, nullptr);
-      wave_sync();
-      c2 += pair_count<kNonAdj>(g, L[5], n5, L[4], n4);
-      wave_sync();
-    }
-    // c1 += f * |D(L2, N(x))| for x = L1[i], f = #{v2 in L0 : x !~ v2}
     for (int32_t i = sub; i < n1; i += nsub) {
-      const int32_t x = L[1][i];
-      const uint32_t f = count_nonadj(g, x, L[0], n0);
-      if (f == 0) continue;
-      const int32_t t = count_prefix<false>(g, x, L[2], n2);
-      c1 += static_cast<uint64_t>(static_cast<int64_t>(t) * f);
+      const int32_t a = L[1][i];
+      const uint32_t q = adj_in<false>(g, a, L[0], n0);
+      const uint32_t d2 = adj_in<false>(g, a, L[2], n2);
+      if (lane == 0) {
+        c1 += static_cast<uint64_t>(q) * d2;
+        c2 += static_cast<uint64_t>(n0) * d2;
+      }
+    }
+    for (int32_t i = sub; i < n2; i += nsub) {
+      const int32_t b = L[2][i];
+      const uint32_t q = adj_in<false>(g, b, L[0], n0);
+      const uint32_t d1v = adj_in<false>(g, b, L[1], n1);
+      if (lane == 0) c1 += static_cast<uint64_t>(q) * d1v;
+    }
+    for (int32_t i = sub; i < n0; i += nsub) {
+      const int32_t c = L[0][i];
+      const int32_t ka = wave_inter(g.nb(c), g.deg(c), L[1], n1, kNoUb, L[3]);
+      const int32_t kb = wave_inter(g.nb(c), g.deg(c), L[2], n2, kNoUb, L[4]);
+      wave_sync();
+      if (lane == 0)
+        c1 += static_cast<uint64_t>(n1 - ka) * static_cast<uint64_t>(n2 - kb);
+      c2 += pair_count<kAdj>(g, L[3], ka, L[4], kb);
+      wave_sync();
     }
   } else if constexpr (P == 6) {  // :736-853
+    // sum over c in L0 = I(N0,N1) of the non-adjacent pairs of
+    // (L0 \ A_c) x (L1 \ B_c), L1 = D(N1,N0), A_c = L0 ∩ N(c), B_c = L1 ∩ N(c)
+    // (c itself stays in L0 \ A_c, as D(L0, N(c)) keeps it).  Expanding the
+    // cross edges as for P5: per c in L0: |L0\A_c| |L1\B_c| + |A_c| |B_c|
+    // - n0 |B_c| - E(A_c,B_c); per b in L1: |N(b) ∩ L0|^2.  P6 reports c1
+    // alone (sm_multigpu.cu:209-232), so the subtracted terms come off c1
+    // (exact modulo 2^64).
     const int32_t n0 = wave_inter(N0, d0, N1, d1, kNoUb, L[0]);
     const int32_t n1 = wave_diff(N1, d1, N0, d0, kNoUb, L[1]);
     wave_sync();
     for (int32_t i = sub; i < n0; i += nsub) {
       const int32_t c = L[0][i];
-      int32_t n2, n3, unused;
-      split_by(g, c, L[0], n0, L[2], n2, nullptr, unused, nullptr);
-      split_by(g, c, L[1], n1, L[3], n3, nullptr, unused, nullptr);
+      const int32_t ka = wave_inter(g.nb(c), g.deg(c), L[0], n0, kNoUb, L[2]);
+      const int32_t kb = wave_inter(g.nb(c), g.deg(c), L[1], n1, kNoUb, L[3]);
       wave_sync();
-      c1 += pair_count<kNonAdj>(g, L[2], n2, L[3], n3);
+      if (lane == 0) {
+        c1 += static_cast<uint64_t>(n0 - ka) * static_cast<uint64_t>(n1 - kb) +
+              static_cast<uint64_t>(ka) * static_cast<uint64_t>(kb);
+        c1 -= static_cast<uint64_t>(n0) * static_cast<uint64_t>(kb);
+      }
+      c1 -= pair_count<kAdj>(g, L[2], ka, L[3], kb);
       wave_sync();
+    }
+    for (int32_t i = sub; i < n1; i += nsub) {
+      const uint64_t q = adj_in<false>(g, L[1][i], L[0], n0);
+      if (lane == 0) c1 += q * q;
     }
   } else if constexpr (P == 4) {  // :855-963, v1 < v0 only
     if (v1 >= v0) return;

@@ -82,29 +82,27 @@ struct EdgeCost {
 
 EdgeCost edge_cost(int p, uint64_t d0, uint64_t d1) {
   const uint64_t m = std::min(d0, d1);
+  const int64_t big = static_cast<int64_t>(std::max(d0, d1));
   switch (p) {
     case 1:  // pairs of I<(N0,N1) x D<(N0,N1); outer = the shorter side
       return {d0 * m + d0 + d1, static_cast<int64_t>(m)};
-    case 2:  // for z in D(N1,N0): split D(N0,N1), pairs of its two parts;
-             // then for y in D(N0,N1): a prefix count
-      return {d1 * (d0 * d0 / 4 + d0) + d0 * d0 + d1,
-              static_cast<int64_t>(std::max(d0, d1))};
-    case 3:  // for y in D(N0,N1): scan it, pairs of its two parts
-      return {d0 * (d0 * d0 / 4 + d0) + d1, static_cast<int64_t>(d0)};
+    case 2:  // per a in D(N0,N1): two counts; per z in D(N1,N0): an
+             // intersection and the edges inside it
+      return {(d0 + d1) * (d0 + 64), big};
+    case 3:  // per v in D(N0,N1): an intersection and the edges above v
+      return {d0 * (d0 + 64) + d1, static_cast<int64_t>(d0)};
     case 4:  // pairs of D(N0,N1) x D(N1,N0)
       return {d0 * d1 + d0 + d1, static_cast<int64_t>(m)};
-    case 5:  // for c in I: split both differences, pairs across them; then
-             // for a in D(N0,N1): two counts
-      return {m * (d0 * d1 + d0 + d1) + d0 * (m + d1),
-              static_cast<int64_t>(d0)};
-    case 6:  // for c in I: split I and D(N1,N0), pairs across them
-      return {m * (m * d1 + m + d1) + d0 + d1, static_cast<int64_t>(m)};
-    case 7:  // P3's machinery on I(N0,N1)
+    case 5:  // per element of I, D(N0,N1), D(N1,N0): counts; per c in I two
+             // intersections and the edges between them
+      return {(d0 + d1) * (m + 64), big};
+    case 6:  // per c in I: two intersections and the edges between them;
+             // per b in D(N1,N0): a count
+      return {(m + d1) * (m + 64), static_cast<int64_t>(std::max(m, d1))};
+    case 7:  // for y in I(N0,N1): scan it, pairs of its two parts
       return {m * (m * m / 4 + m) + d0 + d1, static_cast<int64_t>(m)};
-    default:  // 8: for z in D(N1,N0): split I, pairs of its parts; then
-              // for y in I: two counts
-      return {d1 * (m * m / 4 + m) + m * (d1 + m) + d0,
-              static_cast<int64_t>(std::max(d1, m))};
+    default:  // 8: as P2 on I(N0,N1)
+      return {(m + d1) * (m + 64), static_cast<int64_t>(std::max(d1, m))};
   }
 }
 
