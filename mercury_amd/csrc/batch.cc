@@ -322,6 +322,10 @@ int mgpm_batch_run(mgpm_batch *b, mgpm_query *queries, int nq, void *stream,
       prep = ms_since(tp);
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
+    // A recorded graph replays on the caller's stream, or on the batch's
+    // own stream when the caller passes the default stream (the call is
+    // synchronous either way).
+    if (b->gexec && !s) s = b->cap_stream;
     MGPM_HIP(hipEventRecord(b->ev0, s));
     if (b->gexec) {
       MGPM_HIP(hipGraphLaunch(b->gexec, s));
