@@ -173,6 +173,43 @@ def bench_batch(ma, args) -> int:
     return 0 if counts == single else 1
 
 
+def bench_dag(ma, args) -> int:
+    """Degree-oriented DAG + small/large division of the TC workload's graph
+    (graph.cc:196-321), built on the device vs. on the host, arrays compared."""
+    import numpy as np
+    kind, scale, ef, seed = WORKLOADS[args.workload]
+    g = make_graph(ma, kind, scale, ef, seed)
+    dg = ma.DeviceGraph(g, 0)
+    for _ in range(max(1, args.warmup)):
+        got, st = dg.orientation(128)
+    kms = []
+    for _ in range(args.steps):
+        got, st = dg.orientation(128)
+        kms.append(st.kernel_ms)
+    t0 = time.perf_counter()
+    want = g.orientation(128)
+    host_s = time.perf_counter() - t0
+    same = all(np.array_equal(got[k], want[k]) for k in want)
+    dev_ms = sorted(kms)[len(kms) // 2]
+    m = int(st.m)
+    out = {
+        "metric": "DAG + division arcs/s", "value": 2 * m / (dev_ms / 1e3),
+        "unit": "arcs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dev_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: degree-oriented DAG and "
+                               "small/large division (threshold 128) of the "
+                               "undirected CSR",
+                   "n": int(st.n), "m": m},
+        "device_ms": dev_ms, "host_s": host_s,
+        "host_threads": int(os.environ.get("OMP_NUM_THREADS", "0") or 0),
+        "speedup_vs_host": host_s * 1e3 / dev_ms, "arrays_equal": same,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if same else 1
+
+
 def bench_serve(ma, args) -> int:
     import numpy as np
     names = [p.strip() for p in args.serve_patterns.split(",")]
@@ -249,7 +286,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tc_rmat22", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--mode", default="tc", choices=("tc", "batch", "serve"),
+    ap.add_argument("--mode", default="tc",
+                    choices=("tc", "batch", "serve", "dag"),
                     help="tc: one graph's triangle count (default); batch: "
                          "many graphs' queries in one launch vs one launch "
                          "per query; serve: Poisson request stream through "
@@ -298,6 +336,8 @@ def main() -> int:
         return bench_batch(ma, args)
     if args.mode == "serve":
         return bench_serve(ma, args)
+    if args.mode == "dag":
+        return bench_dag(ma, args)
     kind, scale, ef, seed = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     g = make_graph(ma, kind, scale, ef, seed)

@@ -169,6 +169,19 @@ int mgpm_graph_orientation(const mgpm_graph *g, int32_t deg_thd,
                            int32_t *small_colidx, int64_t *large_rowptr,
                            int32_t *large_colidx);
 
+/* The same arrays built on the device from a device graph's CSR (one wave
+ * per vertex classifies the arcs, a device-wide scan gives the row offsets,
+ * an ordered compaction writes the rows), bit-equal to
+ * mgpm_graph_orientation.  The device arrays stay with the handle; any
+ * non-NULL host array receives a copy (row pointers n+1 int64, column
+ * arrays of the reported sizes).  stats->kernel_ms is the device time of
+ * the build (hipEvents on `stream`), without the copies. */
+int mgpm_dgraph_orientation(mgpm_dgraph *dg, int32_t deg_thd, void *stream,
+                            int64_t sizes[3], int64_t *dag_rowptr,
+                            int32_t *dag_colidx, int64_t *small_rowptr,
+                            int32_t *small_colidx, int64_t *large_rowptr,
+                            int32_t *large_colidx, mgpm_stats *stats);
+
 /* ---- devices and device graphs -------------------------------------------
  * Reference: graph_gpu.h:87-188.  Uploads the undirected CSR (P1..P8) and
  * builds + uploads the triangle-count plan (DAG and table/stream tasks). */

@@ -351,7 +351,7 @@ int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
 int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
   if (!dg || !bytes) return fail(MGPM_ERR_INVALID, "null argument");
   *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->pat.bytes() +
-           dg->scratch.bytes;
+           dg->dag.bytes() + dg->scratch.bytes;
   return MGPM_OK;
 }
 

@@ -113,6 +113,21 @@ struct PatternResident {
   }
 };
 
+// Device-built DAG and small/large division (mgpm_dgraph_orientation).
+struct DagResident {
+  bool ready = false;
+  int32_t deg_thd = 0;
+  int64_t sizes[3] = {0, 0, 0};  // DAG, small, large arcs
+  DevBuf rowptr[3], col[3];       // outputs
+  DevBuf deg, nd, src, val, pos, scan;  // work arrays
+  size_t bytes() const {
+    size_t b = deg.bytes + nd.bytes + src.bytes + val.bytes + pos.bytes +
+               scan.bytes;
+    for (int k = 0; k < 3; ++k) b += rowptr[k].bytes + col[k].bytes;
+    return b;
+  }
+};
+
 // Shared by the C-ABI units.
 TcPlanParams tc_params_from(const mgpm_tune *t);
 int set_device(int device);
@@ -140,6 +155,7 @@ struct mgpm_dgraph {
   mgpm::TcResident tc;
   mgpm::CsrResident csr;
   mgpm::PatternResident pat;
+  mgpm::DagResident dag;
   mgpm::DevBuf scratch;               // queues + result counters
   unsigned long long *pinned = nullptr;  // host mirror of results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -115,6 +115,8 @@ def _load() -> C.CDLL:
                                     C.POINTER(Stats)]),
         "mgpm_pattern_count": (C.c_int, [P, C.c_int, C.POINTER(Tune), P,
                                          C.POINTER(u64), C.POINTER(Stats)]),
+        "mgpm_dgraph_orientation": (C.c_int, [P, i32, P, P, P, P, P, P, P, P,
+                                              C.POINTER(Stats)]),
         "mgpm_autotune": (C.c_int, [P, C.c_int, i32, C.POINTER(Tune),
                                     C.POINTER(C.c_double)]),
         "mgpm_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchOpts), pp]),
@@ -148,6 +150,7 @@ EXPORTED = [
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
     "mgpm_dgraph_upload", "mgpm_dgraph_footprint", "mgpm_dgraph_free",
     "mgpm_tc_count", "mgpm_pattern_count", "mgpm_autotune",
+    "mgpm_dgraph_orientation",
     "mgpm_batch_create",
     "mgpm_batch_run", "mgpm_batch_footprint", "mgpm_batch_free",
     "mgpm_server_create", "mgpm_server_add_graph", "mgpm_server_submit",
@@ -321,6 +324,34 @@ class DeviceGraph:
                                      C.c_void_p(stream or 0), C.byref(cnt),
                                      C.byref(st)), "mgpm_pattern_count")
         return cnt.value, st
+
+    def orientation(self, deg_thd: int = 128,
+                    stream: Optional[int] = None) -> tuple[dict, Stats]:
+        """DAG and small/large division built on the device (the arrays of
+        Graph.orientation), copied back; with the device build's Stats."""
+        import numpy as np
+        sizes = (C.c_int64 * 3)()
+        st = Stats()
+        check(lib.mgpm_dgraph_orientation(self._h, deg_thd,
+                                          C.c_void_p(stream or 0), sizes,
+                                          None, None, None, None, None, None,
+                                          C.byref(st)),
+              "mgpm_dgraph_orientation")
+        n1 = st.n + 1
+        out = {}
+        for k, name in enumerate(("dag", "small", "large")):
+            out[name + "_rowptr"] = np.empty(n1, dtype=np.int64)
+            out[name + "_colidx"] = np.empty(max(sizes[k], 1), dtype=np.int32)
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        check(lib.mgpm_dgraph_orientation(
+            self._h, deg_thd, C.c_void_p(stream or 0), sizes,
+            ptr(out["dag_rowptr"]), ptr(out["dag_colidx"]),
+            ptr(out["small_rowptr"]), ptr(out["small_colidx"]),
+            ptr(out["large_rowptr"]), ptr(out["large_colidx"]), C.byref(st)),
+            "mgpm_dgraph_orientation")
+        for k, name in enumerate(("dag", "small", "large")):
+            out[name + "_colidx"] = out[name + "_colidx"][:sizes[k]]
+        return out, st
 
     def autotune(self, pattern: int = 0, reps: int = 0) -> tuple[Tune, float]:
         """Times candidate knobs on the device, keeps the fastest resident
