@@ -139,6 +139,16 @@ int mgpm_graph_load(const char *prefix, mgpm_graph **out);
 /* Copies the arrays.  Validated like mgpm_graph_load. */
 int mgpm_graph_from_csr(int32_t n, int64_t nnz, const int64_t *rowptr,
                         const int32_t *colidx, mgpm_graph **out);
+/* Edge files, in place of the reference's closed datasets/converter
+ * (converter.sh:5, "converter mtx <in> <out>/graph 1 0 0 0") and
+ * datasets/tsv2mtx.py.  format: 0 = by extension (".mtx" -> 1, else 2),
+ * 1 = MatrixMarket coordinate, 2 = edge list, one "src dst [...]" per line
+ * (GraphChallenge TSV).  base: id of the first vertex (-1 = 1, as in
+ * MatrixMarket and GraphChallenge files; 0 for 0-based lists, the
+ * reference's converter_vidfrom0).  Symmetrized; self-loops and duplicates
+ * dropped.  Errors: MGPM_ERR_IO, MGPM_ERR_FORMAT. */
+int mgpm_graph_read_edges(const char *path, int32_t format, int32_t base,
+                          mgpm_graph **out);
 /* Writes the reference format (meta: "n 2m 4 8 1 2 maxdeg 0 0 0"). */
 int mgpm_graph_write(const mgpm_graph *g, const char *prefix);
 int mgpm_graph_info_get(const mgpm_graph *g, mgpm_graph_info *info);

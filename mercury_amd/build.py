@@ -40,7 +40,8 @@ LIB_SOURCES = [
     "patterns.cc", "batch.cc", "autotune.cc", "server.cc", "dag.cc",
     "tc_kernels.hip", "pattern_kernels.hip", "dag_kernels.hip",
 ]
-CLI_SOURCES = {"tc_challenge": "cli_tc_challenge.cc", "sm": "cli_sm.cc"}
+CLI_SOURCES = {"tc_challenge": "cli_tc_challenge.cc", "sm": "cli_sm.cc",
+               "convert": "cli_convert.cc"}
 
 
 def _run(cmd: list[str]) -> None:

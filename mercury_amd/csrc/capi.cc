@@ -193,6 +193,26 @@ int mgpm_graph_load(const char *prefix, mgpm_graph **out) {
   MGPM_GUARD_END
 }
 
+int mgpm_graph_read_edges(const char *path, int32_t format, int32_t base,
+                          mgpm_graph **out) {
+  if (!path || !out) return fail(MGPM_ERR_INVALID, "null argument");
+  if (format < 0 || format > 2 || base < -1)
+    return fail(MGPM_ERR_INVALID, "bad format or base");
+  MGPM_GUARD_BEGIN
+  std::string p(path);
+  if (format == 0) {
+    const bool mtx = p.size() >= 4 && p.compare(p.size() - 4, 4, ".mtx") == 0;
+    format = mtx ? 1 : 2;
+  }
+  if (base < 0) base = 1;
+  auto g = std::make_shared<CsrGraph>();
+  std::string err;
+  int rc = read_edge_file(p, format, base, *g, err);
+  if (rc) return fail(rc, err);
+  return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
 int mgpm_graph_from_csr(int32_t n, int64_t nnz, const int64_t *rowptr,
                         const int32_t *colidx, mgpm_graph **out) {
   if (!out || n < 0 || nnz < 0 || !rowptr || (nnz > 0 && !colidx))

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 #include "mgpm.h"
@@ -424,6 +425,100 @@ void build_division(const CsrGraph &g, const Dag &d, int32_t deg_thd,
       else if (k == 2) out.large_colidx[lo++] = t;
     }
   }
+}
+
+namespace {
+
+// Next unsigned decimal integer in [p, e) on the current line; skips spaces
+// and tabs only.  False if the line has no further number.
+bool next_int(const char *&p, const char *e, int64_t &v) {
+  while (p < e && (*p == ' ' || *p == '\t' || *p == '\r')) ++p;
+  if (p >= e || *p < '0' || *p > '9') return false;
+  int64_t x = 0;
+  while (p < e && *p >= '0' && *p <= '9') {
+    x = x * 10 + (*p - '0');
+    if (x > (int64_t(1) << 40)) return false;
+    ++p;
+  }
+  v = x;
+  return true;
+}
+
+}  // namespace
+
+int read_edge_file(const std::string &path, int format, int base, CsrGraph &g,
+                   std::string &err) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) {
+    err = "Failed to open file: " + path;
+    return MGPM_ERR_IO;
+  }
+  std::string text((std::istreambuf_iterator<char>(in)),
+                   std::istreambuf_iterator<char>());
+  const char *p = text.data(), *end = p + text.size();
+  int64_t line_no = 0;
+  int64_t n_decl = -1, entries = -1;
+  int64_t max_id = -1;
+  std::vector<uint64_t> pairs;
+  bool banner = false;
+  while (p < end) {
+    const char *eol = static_cast<const char *>(memchr(p, '\n', end - p));
+    if (!eol) eol = end;
+    ++line_no;
+    const char *q = p;
+    p = eol + 1;
+    while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+    if (q >= eol) continue;  // blank line
+    if (format == 1 && line_no == 1) {
+      banner = (eol - q) >= 14 && std::string(q, 14) == "%%MatrixMarket";
+      if (!banner) {
+        err = path + ": missing %%MatrixMarket banner";
+        return MGPM_ERR_FORMAT;
+      }
+      continue;
+    }
+    if (*q == '%' || (format == 2 && *q == '#')) continue;
+    int64_t a = 0, b = 0, c = 0;
+    if (!next_int(q, eol, a) || !next_int(q, eol, b)) {
+      err = path + ":" + std::to_string(line_no) + ": expected two vertex ids";
+      return MGPM_ERR_FORMAT;
+    }
+    if (format == 1 && n_decl < 0) {  // the size line
+      if (!next_int(q, eol, c)) {
+        err = path + ":" + std::to_string(line_no) + ": expected rows cols entries";
+        return MGPM_ERR_FORMAT;
+      }
+      n_decl = std::max(a, b);
+      entries = c;
+      pairs.reserve(static_cast<size_t>(std::min<int64_t>(c, int64_t(1) << 32)));
+      continue;
+    }
+    a -= base;
+    b -= base;
+    const int64_t lim = format == 1 ? n_decl : (int64_t(1) << 31) - 1;
+    if (a < 0 || b < 0 || a >= lim || b >= lim) {
+      err = path + ":" + std::to_string(line_no) + ": vertex id out of range";
+      return MGPM_ERR_FORMAT;
+    }
+    max_id = std::max(max_id, std::max(a, b));
+    pairs.push_back((uint64_t(a) << 32) | uint64_t(b));
+  }
+  if (format == 1 && n_decl < 0) {
+    err = path + ": no size line";
+    return MGPM_ERR_FORMAT;
+  }
+  if (format == 1 && entries >= 0 && static_cast<int64_t>(pairs.size()) != entries) {
+    err = path + ": " + std::to_string(pairs.size()) + " entries, size line says " +
+          std::to_string(entries);
+    return MGPM_ERR_FORMAT;
+  }
+  const int64_t n = format == 1 ? n_decl : max_id + 1;
+  if (n >= (int64_t(1) << 31)) {
+    err = path + ": too many vertices for int32 ids";
+    return MGPM_ERR_FORMAT;
+  }
+  csr_from_draws(static_cast<int32_t>(std::max<int64_t>(n, 0)), pairs, g);
+  return MGPM_OK;
 }
 
 }  // namespace mgpm

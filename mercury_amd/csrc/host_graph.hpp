@@ -30,6 +30,17 @@ struct CsrGraph {
 int read_graph(const std::string &prefix, CsrGraph &g, std::string &err);
 int write_graph(const std::string &prefix, const CsrGraph &g, std::string &err);
 
+// Edge files (in place of the reference's closed datasets/converter and
+// datasets/tsv2mtx.py): format 1 = MatrixMarket coordinate ("%%MatrixMarket"
+// banner, '%' comments, a "rows cols entries" line, then "i j [value]"
+// lines), format 2 = an edge list with one "src dst [...]" per line
+// (GraphChallenge TSV; '#' and '%' lines are comments).  Vertex ids start at
+// `base` (MatrixMarket and GraphChallenge files use 1).  The graph is
+// symmetrized; self-loops and duplicates are dropped.  Returns an MGPM
+// status (MGPM_ERR_IO, MGPM_ERR_FORMAT) with `err` set.
+int read_edge_file(const std::string &path, int format, int base, CsrGraph &g,
+                   std::string &err);
+
 // Checks the invariants the counting kernels rely on (the reference assumes
 // them without checking): offsets monotone and in range, ids in [0,n), rows
 // strictly increasing (sorted, no duplicates), no self-loops, symmetric.

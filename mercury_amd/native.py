@@ -97,6 +97,7 @@ def _load() -> C.CDLL:
         "mgpm_abi_version": (C.c_int, []),
         "mgpm_graph_load": (C.c_int, [C.c_char_p, pp]),
         "mgpm_graph_from_csr": (C.c_int, [i32, i64, P, P, pp]),
+        "mgpm_graph_read_edges": (C.c_int, [C.c_char_p, i32, i32, pp]),
         "mgpm_graph_write": (C.c_int, [P, C.c_char_p]),
         "mgpm_graph_info_get": (C.c_int, [P, C.POINTER(GraphInfo)]),
         "mgpm_graph_csr": (C.c_int, [P, pp, pp]),
@@ -144,7 +145,8 @@ lib = _load()
 # Every symbol include/mgpm.h declares (checked by the CPU test suite).
 EXPORTED = [
     "mgpm_strerror", "mgpm_abi_version", "mgpm_last_error",
-    "mgpm_graph_load", "mgpm_graph_from_csr", "mgpm_graph_write",
+    "mgpm_graph_load", "mgpm_graph_from_csr", "mgpm_graph_read_edges",
+    "mgpm_graph_write",
     "mgpm_graph_info_get", "mgpm_graph_csr", "mgpm_graph_free",
     "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
@@ -197,6 +199,16 @@ class Graph:
         h = C.c_void_p()
         check(lib.mgpm_graph_load(str(prefix).encode(), C.byref(h)),
               f"mgpm_graph_load({prefix})")
+        return cls(h)
+
+    @classmethod
+    def read_edges(cls, path: str, fmt: int = 0, base: int = -1) -> "Graph":
+        """MatrixMarket (fmt 1) or edge-list (fmt 2; 0 = by extension) file;
+        ids start at `base` (-1 = 1)."""
+        h = C.c_void_p()
+        check(lib.mgpm_graph_read_edges(str(path).encode(), fmt, base,
+                                        C.byref(h)),
+              f"mgpm_graph_read_edges({path})")
         return cls(h)
 
     @classmethod
