@@ -131,15 +131,20 @@ __device__ __forceinline__ uint32_t stream_partners(
   return hits;
 }
 
-// Open-addressing hash set in LDS, linear probing, empty = -1.  Keys are
-// vertex ids (>= 0), so an idle item (w = -2) never matches and its probe
-// ends at the first empty slot (tables are at most half full).
+// Open-addressing hash set in LDS, empty = -1.  Keys are vertex ids (>= 0),
+// so an idle item (w = -2) never matches and its probe ends at the first
+// empty slot (tables are at most half full).
 //
 // JOINT: all items of a lane probe together -- each round advances every
 // item still probing and issues all their reads at once (finished items
 // re-read their own slot), so a round costs one LDS round trip however
-// many items miss.  Otherwise each item runs its own probe loop, which is
-// cheaper when tables are tiny and a second probe is rare.
+// many items miss, and the rounds run as long as the longest probe among
+// the wave's items.  To keep that maximum short the sequence is quadratic
+// (triangular offsets h, h+1, h+3, h+6, ..., which visit every slot of a
+// power-of-two table), breaking up the clusters of linear probing; the
+// table must be filled with the same sequence (table_insert, quad).
+// Otherwise (tiny tables, where a second probe is rare) each item runs its
+// own linear probe loop.
 template <bool JOINT>
 struct HashProbe {
   const int32_t *table;
@@ -157,14 +162,16 @@ struct HashProbe {
     }
     uint32_t hits = 0;
     if constexpr (JOINT) {
+      uint32_t step = 1;  // triangular probe sequence h, h+1, h+3, h+6, ...
       for (;;) {
         bool more = false;
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
           const bool miss = s[i] != w[i] && s[i] != -1;
-          h[i] = miss ? ((h[i] + 1) & mask) : h[i];
+          h[i] = miss ? ((h[i] + step) & mask) : h[i];
           more |= miss;
         }
+        ++step;
         if (!more) break;
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) s[i] = table[h[i]];
@@ -212,15 +219,20 @@ struct SortedProbe {
 
 // Builds the hash set of col[tbeg, tbeg+tlen) into table[0, 2^logp) with
 // `nthreads` cooperating threads of index `tid`.  Caller clears first.
+// `quad`: the triangular probe sequence of HashProbe<true>, else linear.
 __device__ __forceinline__ void table_insert(int32_t *table, int logp,
                                              const int32_t *__restrict__ col,
                                              uint32_t tbeg, int tlen, int tid,
-                                             int nthreads) {
+                                             int nthreads, bool quad) {
   const uint32_t mask = (1u << logp) - 1u;
   for (int i = tid; i < tlen; i += nthreads) {
     const int32_t key = col[tbeg + i];
     uint32_t h = slot_of(key, logp);
-    while (atomicCAS(&table[h], -1, key) != -1) h = (h + 1) & mask;
+    uint32_t step = 1;
+    while (atomicCAS(&table[h], -1, key) != -1) {
+      h = (h + (quad ? step : 1u)) & mask;
+      ++step;
+    }
   }
 }
 
@@ -253,7 +265,8 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
   const int P = 1 << logp;
   for (int i = lane; i < P; i += 64) table[i] = -1;
   wave_sync();
-  table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64);
+  table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64,
+               logp_max > 8);
   wave_sync();
   // Tables of up to 256 slots (small owners) probe item by item.
   if (logp_max <= 8) {

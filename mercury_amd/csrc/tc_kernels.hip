@@ -91,7 +91,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     for (int i = threadIdx.x; i < P; i += WAVES * 64) s_table[i] = -1;
     __syncthreads();
     table_insert(s_table, logp, col, task.tbeg, task.tlen, threadIdx.x,
-                 WAVES * 64);
+                 WAVES * 64, true);
     __syncthreads();
     HashProbe<true> probe{s_table, logp, static_cast<uint32_t>(P - 1)};
     acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
