@@ -109,6 +109,17 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
     if (work[a] != work[b]) return work[a] > work[b];
     return a < b;
   });
+  // Queue-entry size per class: at most group_work, and small enough that
+  // the class yields about kTcTargetGroups entries (enough waves to fill
+  // the device) -- small graphs otherwise give a few waves long serial
+  // groups.
+  int64_t class_work[kTcNumClasses] = {0, 0, 0, 0};
+  for (size_t i = 0; i < all.size(); ++i) class_work[cls(all[i].tlen)] += work[i];
+  int64_t limit[kTcNumClasses];
+  for (int c = 0; c < kTcNumClasses; ++c)
+    limit[c] = std::max<int64_t>(
+        kTcMinGroupWork,
+        std::min<int64_t>(p.group_work, class_work[c] / kTcTargetGroups));
   int64_t group_cost[kTcNumClasses] = {0, 0, 0, 0};
   for (int c = 0; c < kTcNumClasses; ++c) {
     plan.tasks[c].clear();
@@ -121,7 +132,7 @@ int build_tc_plan(const CsrGraph &g, const TcPlanParams &p, TcPlan &plan) {
     // Close the open group of class c if this task would overflow it.
     std::vector<int32_t> &go = plan.group_off[c];
     const int32_t open = static_cast<int32_t>(plan.tasks[c].size()) - go.back();
-    if (open > 0 && (group_cost[c] + work[i] > p.group_work || open >= 64)) {
+    if (open > 0 && (group_cost[c] + work[i] > limit[c] || open >= 64)) {
       go.push_back(static_cast<int32_t>(plan.tasks[c].size()));
       plan.group_cost[c].push_back(group_cost[c]);
       group_cost[c] = 0;

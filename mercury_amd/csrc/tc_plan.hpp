@@ -48,6 +48,10 @@ enum TcClass : int {
 // each task's table build kTcTableCost plus its size.
 constexpr int64_t kTcPartnerCost = 16;
 constexpr int64_t kTcTableCost = 64;
+// Queue entries of a class aim at this many (about 16 waves per CU of a
+// 256-CU device), and are never cut below kTcMinGroupWork of cost.
+constexpr int64_t kTcTargetGroups = 4096;
+constexpr int64_t kTcMinGroupWork = 1024;
 
 struct TcPlanParams {
   int32_t small_max = 128;
