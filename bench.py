@@ -57,6 +57,27 @@ def load_traffic(workload: str):
         return None
 
 
+def upload_compare(ma, g, count):
+    """Preparation time of a fresh device graph (CSR upload + DAG + triangle
+    plan), host-built plan against device-built plan, each the second of two
+    uploads (so one-time code loading is not counted), and whether the
+    device-built plan gives the same count."""
+    def timed(device_plan):
+        for _ in range(2):
+            t0 = time.perf_counter()
+            dg = ma.DeviceGraph(g, 0, device_plan=device_plan)
+            dt = time.perf_counter() - t0
+            c = dg.tc_count()[0]
+            ks = sorted(dg.tc_count()[1].kernel_ms for _ in range(5))
+            dg.close()
+        return dt, c, ks[2]
+    host_s, _, host_ms = timed(False)
+    dev_s, c, dev_ms = timed(True)
+    return {"prep_host_plan_s": host_s, "prep_device_plan_s": dev_s,
+            "device_plan_count_equal": c == count,
+            "kernel_ms_host_plan": host_ms, "kernel_ms_device_plan": dev_ms}
+
+
 def cpu_baseline(g, count_gpu, m):
     import numpy as np  # noqa: F401
     import oracle
@@ -286,6 +307,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tc_rmat22", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-device-plan", action="store_true",
+                    help="skip the host-plan vs device-plan upload timing")
     ap.add_argument("--mode", default="tc",
                     choices=("tc", "batch", "serve", "dag"),
                     help="tc: one graph's triangle count (default); batch: "
@@ -397,6 +420,8 @@ def main() -> int:
         "gen_s": gen_s,
         "upload_prep_s": upload_s,
     }
+    if not args.no_device_plan:
+        out.update(upload_compare(ma, g, count))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, count, st.m)
     print(json.dumps(out), flush=True)

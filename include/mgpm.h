@@ -179,9 +179,9 @@ int mgpm_graph_orientation(const mgpm_graph *g, int32_t deg_thd,
                            int32_t *small_colidx, int64_t *large_rowptr,
                            int32_t *large_colidx);
 
-/* The same arrays built on the device from a device graph's CSR (one wave
- * per vertex classifies the arcs, a device-wide scan gives the row offsets,
- * an ordered compaction writes the rows), bit-equal to
+/* The same arrays built on the device from a device graph's CSR (flat
+ * passes over the arcs classify them, a device-wide scan gives the row
+ * offsets, an ordered compaction writes the rows), bit-equal to
  * mgpm_graph_orientation.  The device arrays stay with the handle; any
  * non-NULL host array receives a copy (row pointers n+1 int64, column
  * arrays of the reported sizes).  stats->kernel_ms is the device time of
@@ -199,6 +199,15 @@ int mgpm_device_count(int *count);
 int mgpm_device_synchronize(int device);
 int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
                        mgpm_dgraph **out);
+/* mgpm_dgraph_upload with options.  MGPM_UPLOAD_DEVICE_PLAN: build the DAG
+ * and the triangle-count plan on the device from the uploaded CSR (the
+ * host's orientation_with_division and task lists, graph.cc:196-321 and
+ * tc_challenge.cu:20-53, run on the GPU; no host preprocessing).  Counts
+ * are identical; the plan's task cut and grouping follow the host rules in
+ * a parallel form.  Later re-plans (tunes, autotune) stay on the device. */
+#define MGPM_UPLOAD_DEVICE_PLAN 1
+int mgpm_dgraph_upload_ex(const mgpm_graph *g, int device, void *stream,
+                          int32_t flags, mgpm_dgraph **out);
 /* Device bytes held by the handle (admission control uses this). */
 int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes);
 int mgpm_dgraph_free(mgpm_dgraph *dg);

@@ -107,7 +107,7 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
     }
   }
   // Leave the winner resident (plan, items, grid cap).
-  if ((rc = tc_make_resident(dg, tc_params_from(&cur), nullptr))) return rc;
+  if ((rc = tc_replan(dg, tc_params_from(&cur), nullptr))) return rc;
   dg->tc.items = cur.tc_items;
   dg->tc.grid_cap = cur.grid_blocks;
   dg->tc.cursor_min = cur.tc_cursor_min;
@@ -183,7 +183,7 @@ int autotune_size_classes(const std::vector<mgpm_dgraph *> &graphs, int reps) {
     const TcPlanParams p = tc_params_from(&t);
     for (size_t k = i + 1; k < j; ++k) {
       mgpm_dgraph *g = todo[k];
-      if ((rc = tc_make_resident(g, p, nullptr))) return rc;
+      if ((rc = tc_replan(g, p, nullptr))) return rc;
       g->tc.items = t.tc_items;
       g->tc.grid_cap = t.grid_blocks;
       g->tc.cursor_min = t.tc_cursor_min;

@@ -85,6 +85,7 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   if (rc) return fail(rc, "triangle plan: a list lies outside the DAG");
   TcResident &r = dg->tc;
   r.params = params;
+  r.on_device = false;
   r.generation++;
   r.s_plus = plan.s_plus;
   r.probes = plan.probes;
@@ -110,6 +111,11 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   // The host vectors die with `plan`: wait for the copies.
   MGPM_HIP(hipStreamSynchronize(s));
   return MGPM_OK;
+}
+
+int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s) {
+  return dg->tc.on_device ? tc_make_resident_device(dg, params, s)
+                          : tc_make_resident(dg, params, s);
 }
 
 namespace {
@@ -335,7 +341,14 @@ int mgpm_device_synchronize(int device) {
 
 int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
                        mgpm_dgraph **out) {
+  return mgpm_dgraph_upload_ex(g, device, stream, 0, out);
+}
+
+int mgpm_dgraph_upload_ex(const mgpm_graph *g, int device, void *stream,
+                          int32_t flags, mgpm_dgraph **out) {
   if (!g || !out) return fail(MGPM_ERR_INVALID, "null argument");
+  if (flags & ~MGPM_UPLOAD_DEVICE_PLAN)
+    return fail(MGPM_ERR_INVALID, "unknown upload flags");
   MGPM_GUARD_BEGIN
   int rc = set_device(device);
   if (rc) return rc;
@@ -361,7 +374,9 @@ int mgpm_dgraph_upload(const mgpm_graph *g, int device, void *stream,
   if (rc) return rc;
   rc = upload_vec(dg->csr.colidx, G.colidx, s);
   if (rc) return rc;
-  rc = tc_make_resident(dg.get(), tc_params_from(nullptr), s);
+  rc = (flags & MGPM_UPLOAD_DEVICE_PLAN)
+           ? tc_make_resident_device(dg.get(), tc_params_from(nullptr), s)
+           : tc_make_resident(dg.get(), tc_params_from(nullptr), s);
   if (rc) return rc;
   *out = dg.release();
   return MGPM_OK;
@@ -402,7 +417,7 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   const TcPlanParams want = tune ? tc_params_from(tune) : dg->tc.params;
   if (!same_params(want, dg->tc.params)) {
     const auto tp = std::chrono::steady_clock::now();
-    rc = tc_make_resident(dg, want, s);
+    rc = tc_replan(dg, want, s);
     if (rc) return rc;
     prep = ms_since(tp);
   }

@@ -72,6 +72,7 @@ struct TcResident {
   int grid_cap = 0;         // cap on persistent workgroups (0 = none)
   int cursor_min = -1;      // owner-cursor streaming threshold (<= 0: off)
   bool tuned = false;       // knobs chosen by the autotuner
+  bool on_device = false;   // plan built by tc_make_resident_device
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
   int32_t max_out = 0;
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
@@ -133,6 +134,11 @@ TcPlanParams tc_params_from(const mgpm_tune *t);
 int set_device(int device);
 int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s);
+// The same plan built on the device from the resident CSR (tc_dplan.cc).
+int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
+                            hipStream_t s);
+// Rebuilds the resident triangle plan where it was built (host or device).
+int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s);
 // Builds pattern `pattern`'s plan if missing or if `split` (0 = the
 // graph's current setting) differs from the resident plan's.
 int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int pattern,

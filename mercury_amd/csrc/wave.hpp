@@ -59,6 +59,15 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(
   return x;
 }
 
+__device__ __forceinline__ int wave_max_i32(int x) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    const int y = __shfl_xor(x, d, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
 __device__ __forceinline__ int wave_sum_i32(int x) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);

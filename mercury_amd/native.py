@@ -110,6 +110,7 @@ def _load() -> C.CDLL:
         "mgpm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
         "mgpm_device_synchronize": (C.c_int, [C.c_int]),
         "mgpm_dgraph_upload": (C.c_int, [P, C.c_int, P, pp]),
+        "mgpm_dgraph_upload_ex": (C.c_int, [P, C.c_int, P, i32, pp]),
         "mgpm_dgraph_footprint": (C.c_int, [P, C.POINTER(u64)]),
         "mgpm_dgraph_free": (C.c_int, [P]),
         "mgpm_tc_count": (C.c_int, [P, C.POINTER(Tune), P, C.POINTER(u64),
@@ -150,7 +151,8 @@ EXPORTED = [
     "mgpm_graph_info_get", "mgpm_graph_csr", "mgpm_graph_free",
     "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
-    "mgpm_dgraph_upload", "mgpm_dgraph_footprint", "mgpm_dgraph_free",
+    "mgpm_dgraph_upload", "mgpm_dgraph_upload_ex", "mgpm_dgraph_footprint",
+    "mgpm_dgraph_free",
     "mgpm_tc_count", "mgpm_pattern_count", "mgpm_autotune",
     "mgpm_dgraph_orientation",
     "mgpm_batch_create",
@@ -307,12 +309,18 @@ class Graph:
 class DeviceGraph:
     """Device-resident graph (``mgpm_dgraph``) on one GPU."""
 
+    UPLOAD_DEVICE_PLAN = 1
+
     def __init__(self, graph: Graph, device: int = 0,
-                 stream: Optional[int] = None):
+                 stream: Optional[int] = None, device_plan: bool = False):
+        """device_plan: build the DAG and the triangle plan on the device
+        (``MGPM_UPLOAD_DEVICE_PLAN``) instead of on the host."""
         self._h = C.c_void_p()
-        check(lib.mgpm_dgraph_upload(graph._h, device,
-                                     C.c_void_p(stream or 0),
-                                     C.byref(self._h)), "mgpm_dgraph_upload")
+        flags = self.UPLOAD_DEVICE_PLAN if device_plan else 0
+        check(lib.mgpm_dgraph_upload_ex(graph._h, device,
+                                        C.c_void_p(stream or 0), flags,
+                                        C.byref(self._h)),
+              "mgpm_dgraph_upload_ex")
         self.device = device
 
     def footprint(self) -> int:
