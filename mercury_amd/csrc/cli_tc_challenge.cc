@@ -3,10 +3,13 @@
 // Counting", "Runtime  = <s> sec", "total_num_triangles = <n>") and one
 // JSON line with the measurements.  Runtime is the device time of the
 // counting kernels (load, DAG and upload excluded, as tc_challenge.cu:98-110).
+// The DAG and the triangle plan are built on the device
+// (MGPM_UPLOAD_DEVICE_PLAN); `--host-plan` builds them on the host instead.
 #include <chrono>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 
 #include "mgpm.h"
 
@@ -17,8 +20,13 @@ static int die(int rc, const char *what) {
 }
 
 int main(int argc, char **argv) {
+  int32_t flags = MGPM_UPLOAD_DEVICE_PLAN;
+  if (argc >= 2 && std::strcmp(argv[argc - 1], "--host-plan") == 0) {
+    flags = 0;
+    --argc;
+  }
   if (argc < 2) {
-    std::printf("Usage: %s <graph>\n", argv[0]);
+    std::printf("Usage: %s <graph> [device] [--host-plan]\n", argv[0]);
     std::printf("Example: %s /graph_inputs/mico/graph\n", argv[0]);
     return 1;
   }
@@ -33,7 +41,7 @@ int main(int argc, char **argv) {
   std::printf("|V| %d |E| %" PRId64 "\n", gi.n, gi.nnz);
   const auto t1 = std::chrono::steady_clock::now();
   mgpm_dgraph *dg = nullptr;
-  rc = mgpm_dgraph_upload(g, device, nullptr, &dg);
+  rc = mgpm_dgraph_upload_ex(g, device, nullptr, flags, &dg);
   if (rc) return die(rc, "upload");
   const auto t2 = std::chrono::steady_clock::now();
   uint64_t total = 0;

@@ -98,10 +98,12 @@ def test_sm_cli_matches_golden(gpu_available):
         assert int(line.split("=")[1]) == want[f"P{p}"]
 
 
-def test_tc_challenge_cli_matches_golden(gpu_available):
+@pytest.mark.parametrize("plan", [[], ["--host-plan"]])
+def test_tc_challenge_cli_matches_golden(gpu_available, plan):
+    """Default (plan built on the device) and --host-plan."""
     want = _golden()["rmat9"]["counts"]["TC"]
     r = subprocess.run([str(ROOT / "mercury_amd" / "bin" / "tc_challenge"),
-                        str(GOLDEN / "rmat9" / "graph")],
+                        str(GOLDEN / "rmat9" / "graph")] + plan,
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert "Runtime  = " in r.stdout
