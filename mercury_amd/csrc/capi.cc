@@ -91,6 +91,7 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   r.probes = plan.probes;
   r.m_plus = static_cast<uint64_t>(plan.m_plus);
   r.max_out = plan.max_out;
+  if (!r.tuned) r.items = default_tc_items(plan.probes, plan.partners.size());
   if ((rc = upload_vec(r.col, plan.dag_col, s))) return rc;
   if ((rc = upload_vec(r.partners, plan.partners, s))) return rc;
   for (int c = 0; c < kTcNumClasses; ++c) {

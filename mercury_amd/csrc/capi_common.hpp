@@ -137,6 +137,12 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
 // The same plan built on the device from the resident CSR (tc_dplan.cc).
 int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
                             hipStream_t s);
+// Items per lane the counting kernels use when no tune says otherwise:
+// short streamed lists on average (fewer than 32 items per partner, e.g.
+// uniform random graphs) waste most of an 8-row step, so they take 4.
+inline int default_tc_items(uint64_t probes, uint64_t partners) {
+  return probes < 32 * partners ? 4 : 8;
+}
 // Rebuilds the resident triangle plan where it was built (host or device).
 int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s);
 // Builds pattern `pattern`'s plan if missing or if `split` (0 = the

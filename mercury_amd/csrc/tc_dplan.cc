@@ -164,6 +164,7 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   r.probes = static_cast<uint64_t>(cost_total - kTcPartnerCost * np);
   r.m_plus = static_cast<uint64_t>(m);
   r.max_out = sum.max_out;
+  if (!r.tuned) r.items = default_tc_items(r.probes, static_cast<uint64_t>(np));
   TcDevicePlan &p = r.dev;
   p = TcDevicePlan();
   const TcTask *tb = r.tasks[0].as<TcTask>();
