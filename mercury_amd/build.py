@@ -38,8 +38,8 @@ CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-Wall",
 LIB_SOURCES = [
     "host_graph.cc", "tc_plan.cc", "pattern_plan.cc", "capi.cc",
     "patterns.cc", "batch.cc", "autotune.cc", "server.cc", "dag.cc",
-    "tc_dplan.cc", "tc_kernels.hip", "pattern_kernels.hip", "dag_kernels.hip",
-    "tc_dplan_kernels.hip",
+    "tc_dplan.cc", "census.cc", "tc_kernels.hip", "pattern_kernels.hip",
+    "dag_kernels.hip", "tc_dplan_kernels.hip", "census_kernels.hip",
 ]
 CLI_SOURCES = {"tc_challenge": "cli_tc_challenge.cc", "sm": "cli_sm.cc",
                "convert": "cli_convert.cc"}

@@ -1,0 +1,101 @@
+// P1 (induced claws) in closed form from a triangle census on the device.
+//
+// The reference counts P1 per directed edge with frequency arrays
+// (pattern_kernels.cuh:6-133, combined as sm_multigpu.cu:209-232); its sum
+// is the number of induced claws K1,3.  A claw centred at v is an
+// independent triple of N(v), and by inclusion-exclusion over the edges
+// inside N(v):
+//   claws(v) = C(d,3) - t_v (d - 2) + sum_{u in N(v)} C(t_uv, 2) - K4(v)
+// (t_v: triangles at v, t_uv: triangles on edge uv, K4(v): 4-cliques at v).
+// Summed over v:
+//   P1 = sum_v C(d_v,3) - sum_triangles (d_x + d_y + d_z - 6)
+//        + 2 sum_edges C(t_e,2) - 4 K4,
+// which census_kernels.hip computes by one enumeration of the triangles of
+// the degree-oriented DAG.  Exact in uint64 (mod 2^64 in the intermediate
+// terms, like the reference's unsigned counters).
+#include <algorithm>
+#include <cstring>
+
+#include "capi_common.hpp"
+#include "census.hpp"
+#include "dag_device.hpp"
+
+namespace mgpm {
+
+int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
+                         uint64_t *count, float *kernel_ms) {
+  if (pattern != MGPM_P1)
+    return fail(MGPM_ERR_PATTERN, "census: only P1 has a closed form here");
+  const CsrGraph &G = *dg->host;
+  const int64_t nnz = G.nnz;
+  const int32_t n = G.n;
+  const int64_t m = nnz / 2;
+  if (nnz >= (int64_t(1) << 32))
+    return fail(MGPM_ERR_CAPACITY, "census: 2^32 or more arcs");
+  const size_t a1 = static_cast<size_t>(nnz) + 1;
+  DevBuf deg, nd, src, val, pos, drp, dcol, dsrc, t, slab, small, scan;
+  MGPM_HIP(deg.alloc(std::max<int32_t>(n, 1) * sizeof(int32_t)));
+  MGPM_HIP(nd.alloc(std::max<int32_t>(n, 1) * sizeof(int32_t)));
+  MGPM_HIP(src.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  MGPM_HIP(val.alloc(a1 * sizeof(int64_t)));
+  MGPM_HIP(pos.alloc(a1 * sizeof(int64_t)));
+  MGPM_HIP(drp.alloc((static_cast<size_t>(n) + 1) * sizeof(int64_t)));
+  MGPM_HIP(dcol.alloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
+  MGPM_HIP(dsrc.alloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
+  MGPM_HIP(t.alloc(std::max<int64_t>(m, 1) * sizeof(uint32_t)));
+  size_t tb = 0;
+  if (dag_scan_bytes(nnz, &tb)) return fail(MGPM_ERR_HIP, "scan size query");
+  MGPM_HIP(scan.alloc(std::max<size_t>(tb, 1)));
+  // The largest DAG out-degree bounds every triangle set C.
+  const int64_t slab_len = std::max<int32_t>(dg->tc.max_out, 1);
+  const int waves = census_waves(dg->cus);
+  MGPM_HIP(slab.alloc(static_cast<size_t>(waves) * slab_len * sizeof(int32_t)));
+  MGPM_HIP(small.alloc(256));
+
+  DagDevice d;
+  d.rowptr = dg->csr.rowptr.as<int64_t>();
+  d.col = dg->csr.colidx.as<int32_t>();
+  d.n = n;
+  d.nnz = nnz;
+  d.division = false;
+  d.cus = dg->cus;
+  d.deg = deg.as<int32_t>();
+  d.nd = nd.as<int32_t>();
+  d.src = src.as<int32_t>();
+  d.val = val.as<int64_t>();
+  d.pos = pos.as<int64_t>();
+  d.scan_tmp = scan.p;
+  d.scan_bytes = tb;
+  d.out_rowptr[0] = drp.as<int64_t>();
+  d.out_col[0] = dcol.as<int32_t>();
+
+  CensusArgs c;
+  c.drp = d.out_rowptr[0];
+  c.dcol = d.out_col[0];
+  c.dsrc = dsrc.as<int32_t>();
+  c.deg = d.deg;
+  c.n = n;
+  c.m = m;
+  c.t = t.as<uint32_t>();
+  c.slab = slab.as<int32_t>();
+  c.slab_len = slab_len;
+  c.cursor = small.as<unsigned>();
+  c.acc = reinterpret_cast<unsigned long long *>(small.as<char>() + 64);
+
+  MGPM_HIP(hipEventRecord(dg->ev0, s));
+  if (dag_device_build(d, s)) return fail(MGPM_ERR_HIP, "DAG kernel launch failed");
+  MGPM_HIP(hipMemsetAsync(t.p, 0, t.bytes, s));
+  MGPM_HIP(hipMemsetAsync(small.p, 0, small.bytes, s));
+  if (census_launch(c, dg->cus, s))
+    return fail(MGPM_ERR_HIP, "census kernel launch failed");
+  MGPM_HIP(hipEventRecord(dg->ev1, s));
+  unsigned long long acc[kCensusSlots];
+  MGPM_HIP(hipMemcpyAsync(acc, c.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
+  MGPM_HIP(hipStreamSynchronize(s));
+  MGPM_HIP(hipEventElapsedTime(kernel_ms, dg->ev0, dg->ev1));
+  *count = acc[kCensusDeg3] - acc[kCensusTriDegrees] +
+           2 * acc[kCensusEdgePairs] - 4 * acc[kCensusK4];
+  return MGPM_OK;
+}
+
+}  // namespace mgpm

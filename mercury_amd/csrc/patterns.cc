@@ -101,6 +101,19 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
     int rc = set_device(dg->device);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (pattern == MGPM_P1 && !(tune && tune->pattern_census < 0)) {
+      float kms = 0;
+      if ((rc = census_pattern_count(dg, pattern, s, count, &kms))) return rc;
+      if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->kernel_ms = kms;
+        stats->n = static_cast<uint64_t>(dg->host->n);
+        stats->m = static_cast<uint64_t>(dg->host->nnz / 2);
+        stats->path = 30 + pattern;
+        stats->e2e_ms = ms_since(t0);
+      }
+      return MGPM_OK;
+    }
     double prep = 0;
     const int split = tune ? tune->pattern_split : 0;
     const PatternPlanResident &pr = dg->pat.plan[pattern];

@@ -29,7 +29,8 @@ class Tune(C.Structure):
                 ("block_max_deg", C.c_int32), ("chunk_work", C.c_int32),
                 ("grid_blocks", C.c_int32), ("pattern_waves", C.c_int32),
                 ("group_work", C.c_int32), ("tc_items", C.c_int32),
-                ("tc_cursor_min", C.c_int32), ("pattern_split", C.c_int32)]
+                ("tc_cursor_min", C.c_int32), ("pattern_split", C.c_int32),
+                ("pattern_census", C.c_int32)]
 
 
 class Stats(C.Structure):
