@@ -96,8 +96,8 @@ typedef struct mgpm_tune {
                              estimated pair tests exceed 2^pattern_split is
                              cut into shares run by different waves
                              (default 20; -1 = never) */
-  int32_t pattern_census; /* P1: 0 (default) or 1 = closed form from a
-                             triangle census of the DAG; -1 = the
+  int32_t pattern_census; /* P1, P4: 0 (default) or 1 = closed form from
+                             a triangle and 4-cycle census; -1 = the
                              reference's per-edge set algebra */
 } mgpm_tune;
 

@@ -143,7 +143,7 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
 inline int default_tc_items(uint64_t probes, uint64_t partners) {
   return probes < 32 * partners ? 4 : 8;
 }
-// P1 in closed form from a triangle census (census.cc).
+// P1 and P4 in closed form from a triangle and 4-cycle census (census.cc).
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms);
 // Rebuilds the resident triangle plan where it was built (host or device).

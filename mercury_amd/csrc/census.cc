@@ -1,4 +1,5 @@
-// P1 (induced claws) in closed form from a triangle census on the device.
+// P1 (induced claws) and P4 (induced paths on 4 vertices) in closed form
+// from a triangle and 4-cycle census on the device.
 //
 // The reference counts P1 per directed edge with frequency arrays
 // (pattern_kernels.cuh:6-133, combined as sm_multigpu.cu:209-232); its sum
@@ -11,8 +12,18 @@
 //   P1 = sum_v C(d_v,3) - sum_triangles (d_x + d_y + d_z - 6)
 //        + 2 sum_edges C(t_e,2) - 4 K4,
 // which census_kernels.hip computes by one enumeration of the triangles of
-// the degree-oriented DAG.  Exact in uint64 (mod 2^64 in the intermediate
-// terms, like the reference's unsigned counters).
+// the degree-oriented DAG.
+//
+// P4 (pattern_kernels.cuh:855-963, one task per undirected edge) counts,
+// per edge {x,y}, pairs a in N(x)\N[y], b in N(y)\N[x] with a, b not
+// adjacent: (d_x - t - 1)(d_y - t - 1) pairs minus those with a ~ b, which
+// are exactly the induced 4-cycles x-a-b-y through the edge.  Each induced
+// 4-cycle has 4 edges, and 4-cycle subgraphs sit on induced 4-cycles (1),
+// diamonds (1) and K4s (3), with diamonds = sum C(t_e,2) - 6 K4:
+//   P4 = sum_edges (d_x - t - 1)(d_y - t - 1)
+//        - 4 (C4 - sum_edges C(t_e,2) + 3 K4).
+// Exact in uint64 (mod 2^64 in the intermediate terms, like the
+// reference's unsigned counters).
 #include <algorithm>
 #include <cstring>
 
@@ -22,10 +33,13 @@
 
 namespace mgpm {
 
+// Device bytes of the 4-cycle pass's per-workgroup arrays.
+constexpr size_t kCensusC4Budget = size_t(4) << 30;
+
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms) {
-  if (pattern != MGPM_P1)
-    return fail(MGPM_ERR_PATTERN, "census: only P1 has a closed form here");
+  if (pattern != MGPM_P1 && pattern != MGPM_P4)
+    return fail(MGPM_ERR_PATTERN, "census: only P1 and P4 have closed forms");
   const CsrGraph &G = *dg->host;
   const int64_t nnz = G.nnz;
   const int32_t n = G.n;
@@ -51,6 +65,17 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
   const int waves = census_waves(dg->cus);
   MGPM_HIP(slab.alloc(static_cast<size_t>(waves) * slab_len * sizeof(int32_t)));
   MGPM_HIP(small.alloc(256));
+  // P4 also counts 4-cycles: per workgroup a dense counter array and a
+  // list over all vertices (within a fixed budget).
+  DevBuf wedges, touched;
+  int groups = 0;
+  if (pattern == MGPM_P4) {
+    const size_t per = std::max<size_t>(n, 1) * sizeof(uint32_t);
+    groups = static_cast<int>(std::min<size_t>(
+        1024, std::max<size_t>(64, kCensusC4Budget / (2 * per))));
+    MGPM_HIP(wedges.alloc(groups * per));
+    MGPM_HIP(touched.alloc(groups * per));
+  }
 
   DagDevice d;
   d.rowptr = dg->csr.rowptr.as<int64_t>();
@@ -81,6 +106,11 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
   c.slab_len = slab_len;
   c.cursor = small.as<unsigned>();
   c.acc = reinterpret_cast<unsigned long long *>(small.as<char>() + 64);
+  c.rowptr = d.rowptr;
+  c.col = d.col;
+  c.wedges = wedges.as<uint32_t>();
+  c.touched = touched.as<int32_t>();
+  c.c4_groups = groups;
 
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (dag_device_build(d, s)) return fail(MGPM_ERR_HIP, "DAG kernel launch failed");
@@ -88,13 +118,22 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
   MGPM_HIP(hipMemsetAsync(small.p, 0, small.bytes, s));
   if (census_launch(c, dg->cus, s))
     return fail(MGPM_ERR_HIP, "census kernel launch failed");
+  if (pattern == MGPM_P4) {
+    MGPM_HIP(hipMemsetAsync(wedges.p, 0, wedges.bytes, s));
+    if (census_c4_launch(c, s))
+      return fail(MGPM_ERR_HIP, "census kernel launch failed");
+  }
   MGPM_HIP(hipEventRecord(dg->ev1, s));
   unsigned long long acc[kCensusSlots];
   MGPM_HIP(hipMemcpyAsync(acc, c.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
   MGPM_HIP(hipStreamSynchronize(s));
   MGPM_HIP(hipEventElapsedTime(kernel_ms, dg->ev0, dg->ev1));
-  *count = acc[kCensusDeg3] - acc[kCensusTriDegrees] +
-           2 * acc[kCensusEdgePairs] - 4 * acc[kCensusK4];
+  if (pattern == MGPM_P1)
+    *count = acc[kCensusDeg3] - acc[kCensusTriDegrees] +
+             2 * acc[kCensusEdgePairs] - 4 * acc[kCensusK4];
+  else  // induced 4-cycles = C4 - sum C(t_e,2) + 3 K4
+    *count = acc[kCensusPathPairs] -
+             4 * (acc[kCensusC4] - acc[kCensusEdgePairs] + 3 * acc[kCensusK4]);
   return MGPM_OK;
 }
 

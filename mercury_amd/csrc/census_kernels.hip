@@ -1,5 +1,6 @@
-// Subgraph census by triangle enumeration over the degree-oriented DAG:
-// the terms from which P1 (induced claws) follows in closed form.
+// Subgraph census by triangle enumeration over the degree-oriented DAG,
+// and a 4-cycle count: the terms from which P1 (induced claws) and P4
+// (induced paths on 4 vertices) follow in closed form.
 //
 // One wave per DAG arc x->y (arcs handed out by an atomic cursor, so the
 // arcs of a hub row spread over many waves):
@@ -166,7 +167,74 @@ __global__ void census_sums(CensusArgs c) {
   }
 }
 
+// 4-cycles as subgraphs, each counted once at its highest-ranked vertex v
+// (rank: degree, then id -- the DAG order): for every neighbour u of v
+// ranked below v and every neighbour w of u ranked below v, count the
+// wedge v-u-w at w; a cycle v-u-w-u' is one pair of wedges at w, so v adds
+// sum_w C(wedges(w), 2).  One workgroup per v (vertices handed out by an
+// atomic cursor), with its own dense counter array over all vertices
+// (left zeroed) and the list of the w it touched.
+__device__ __forceinline__ bool rank_below(const int32_t *__restrict__ deg,
+                                           int32_t a, int32_t b) {
+  const int32_t da = deg[a], db = deg[b];
+  return da < db || (da == db && a < b);
+}
+
+constexpr int kC4Threads = 256;
+
+__global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
+  uint32_t *cnt = c.wedges + size_t(blockIdx.x) * size_t(c.n);
+  int32_t *touched = c.touched + size_t(blockIdx.x) * size_t(c.n);
+  __shared__ unsigned s_v, s_nt;
+  __shared__ unsigned long long s_acc[kC4Threads / 64];
+  const int lane = lane64(), wv = threadIdx.x >> 6;
+  constexpr int nwv = kC4Threads / 64;
+  unsigned long long acc = 0;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      s_v = atomicAdd(c.cursor + 1, 1u);
+      s_nt = 0;
+    }
+    __syncthreads();
+    const unsigned v = s_v;
+    if (v >= static_cast<unsigned>(c.n)) break;
+    const int64_t ev = c.rowptr[v + 1];
+    for (int64_t i = c.rowptr[v] + wv; i < ev; i += nwv) {
+      const int32_t u = c.col[i];
+      if (!rank_below(c.deg, u, static_cast<int32_t>(v))) continue;
+      const int64_t eu = c.rowptr[u + 1];
+      for (int64_t j = c.rowptr[u] + lane; j < eu; j += 64) {
+        const int32_t w = c.col[j];
+        if (!rank_below(c.deg, w, static_cast<int32_t>(v))) continue;
+        if (atomicAdd(&cnt[w], 1u) == 0u) touched[atomicAdd(&s_nt, 1u)] = w;
+      }
+    }
+    __syncthreads();
+    const unsigned nt = s_nt;
+    for (unsigned k = threadIdx.x; k < nt; k += kC4Threads) {
+      // Read and clear in one atomic (the counts were made by atomics).
+      const unsigned long long x = atomicExch(&cnt[touched[k]], 0u);
+      acc += x * (x - 1) / 2;
+    }
+    __syncthreads();  // s_v, s_nt and the list are reused
+  }
+  acc = wave_sum_u64(acc);
+  if (lane == 0) s_acc[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < nwv; ++k) t += s_acc[k];
+    if (t) atomicAdd(&c.acc[kCensusC4], t);
+  }
+}
+
 }  // namespace
+
+int census_c4_launch(const CensusArgs &c, hipStream_t s) {
+  hipLaunchKernelGGL(census_c4, dim3(std::max(c.c4_groups, 1)),
+                     dim3(kC4Threads), 0, s, c);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int census_waves(int cus) { return cus * 16; }
 
