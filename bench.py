@@ -231,6 +231,54 @@ def bench_dag(ma, args) -> int:
     return 0 if same else 1
 
 
+def bench_patterns(ma, args) -> int:
+    """P1..P8 counts on one RMAT graph (default RMAT-20, ef 16, seed 4, the
+    pattern configuration): device time per count for each requested
+    pattern and method ("census": P1/P4 closed forms; "kernels": the
+    reference's per-edge set algebra), and whether the methods agree."""
+    scale, ef, seed = args.pattern_graph
+    names = [p.strip() for p in args.patterns.split(",")]
+    methods = [m.strip() for m in args.pattern_methods.split(",")]
+    t0 = time.perf_counter()
+    g = ma.Graph.rmat(scale, ef, seed=seed)
+    dg = ma.DeviceGraph(g, 0)
+    prep_s = time.perf_counter() - t0
+    rows, agree = {}, True
+    for name in names:
+        p = ma.PATTERN_NAMES[name]
+        counts = {}
+        for meth in methods:
+            if meth == "census" and p not in (1, 4):
+                continue
+            tune = ma.make_tune(pattern_census=-1) if meth == "kernels" else None
+            dg.pattern_count(p, tune)  # builds plans outside the timing
+            ks = []
+            for _ in range(args.steps):
+                c, st = dg.pattern_count(p, tune)
+                ks.append(st.kernel_ms)
+            print(f"# {name} {meth}: {c} in {sorted(ks)[len(ks) // 2]:.1f} ms",
+                  file=sys.stderr, flush=True)
+            counts[meth] = c
+            rows[f"{name}_{meth}_ms"] = sorted(ks)[len(ks) // 2]
+            rows[f"{name}_count"] = c
+        agree = agree and len(set(counts.values())) <= 1
+    first = f"{names[0]}_{methods[0] if names[0] in ('P1', 'P4') else 'kernels'}_ms"
+    out = {
+        "metric": "pattern count time", "value": rows.get(first),
+        "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": rows.get(first), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"patterns {','.join(names)} on RMAT scale "
+                               f"{scale}, edge factor {ef}, seed {seed}",
+                   "n": int(g.n), "m": int(g.nnz) // 2,
+                   "methods": methods},
+        "methods_agree": agree, "prep_s": prep_s, **rows,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if agree else 1
+
+
 def bench_serve(ma, args) -> int:
     import numpy as np
     names = [p.strip() for p in args.serve_patterns.split(",")]
@@ -307,10 +355,16 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="tc_rmat22", choices=WORKLOADS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pattern-graph", type=int, nargs=3, default=(20, 16, 4),
+                    metavar=("SCALE", "EF", "SEED"),
+                    help="patterns mode: the RMAT graph")
+    ap.add_argument("--patterns", default="P1,P4")
+    ap.add_argument("--pattern-methods", default="census",
+                    help="patterns mode: census and/or kernels")
     ap.add_argument("--no-device-plan", action="store_true",
                     help="skip the host-plan vs device-plan upload timing")
     ap.add_argument("--mode", default="tc",
-                    choices=("tc", "batch", "serve", "dag"),
+                    choices=("tc", "batch", "serve", "dag", "patterns"),
                     help="tc: one graph's triangle count (default); batch: "
                          "many graphs' queries in one launch vs one launch "
                          "per query; serve: Poisson request stream through "
@@ -361,6 +415,8 @@ def main() -> int:
         return bench_serve(ma, args)
     if args.mode == "dag":
         return bench_dag(ma, args)
+    if args.mode == "patterns":
+        return bench_patterns(ma, args)
     kind, scale, ef, seed = WORKLOADS[args.workload]
     t0 = time.perf_counter()
     g = make_graph(ma, kind, scale, ef, seed)
