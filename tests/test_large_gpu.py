@@ -1,0 +1,43 @@
+"""Full-size workloads against oracle counts recorded in
+tests/golden/large_counts.json (made by tests/golden/make_large_counts.py):
+RMAT-22 and ER-22 triangle counts with the host- and the device-built plan,
+and RMAT-18 P1 / P4 by census and by the per-edge kernels."""
+import json
+from pathlib import Path
+
+import pytest
+
+import mercury_amd as ma
+
+pytestmark = pytest.mark.gpu
+
+LARGE = json.loads((Path(__file__).resolve().parent / "golden" /
+                    "large_counts.json").read_text())
+
+
+@pytest.mark.parametrize("key,make", [
+    ("tc_rmat22", lambda: ma.Graph.rmat(22, 16, seed=2)),
+    ("tc_er22", lambda: ma.Graph.er(1 << 22, 16 << 22, 3)),
+])
+def test_tc_full_size(gpu_available, key, make):
+    g = make()
+    for device_plan in (False, True):
+        dg = ma.DeviceGraph(g, 0, device_plan=device_plan)
+        got = dg.tc_count()[0]
+        dg.close()
+        print(f"{key} device_plan={device_plan}: {got}", flush=True)
+        assert got == LARGE[key]
+
+
+def test_p1_p4_rmat18(gpu_available):
+    dg = ma.DeviceGraph(ma.Graph.rmat(18, 16, seed=4), 0)
+    kern = ma.make_tune(pattern_census=-1)
+    p1 = dg.pattern_count(1)[0]
+    print(f"rmat18 P1 census {p1}", flush=True)
+    assert p1 == LARGE["p1_rmat18"]
+    assert dg.pattern_count(1, kern)[0] == p1
+    p4 = dg.pattern_count(4)[0]
+    p4k = dg.pattern_count(4, kern)[0]
+    print(f"rmat18 P4 census {p4} kernels {p4k}", flush=True)
+    assert p4 == p4k
+    dg.close()
