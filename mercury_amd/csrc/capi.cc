@@ -83,7 +83,36 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   if (rc == MGPM_ERR_CAPACITY)
     return fail(rc, "triangle plan: DAG has 2^32 or more arcs");
   if (rc) return fail(rc, "triangle plan: a list lies outside the DAG");
+  // Upload into fresh buffers and install them only when every copy has
+  // landed, so a failure leaves the resident plan untouched.
+  DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
+  TcDevicePlan p;
+  if ((rc = upload_vec(col, plan.dag_col, s))) return rc;
+  if ((rc = upload_vec(partners, plan.partners, s))) return rc;
+  for (int c = 0; c < kTcNumClasses; ++c) {
+    if ((rc = upload_vec(tasks[c], plan.tasks[c], s))) return rc;
+    if ((rc = upload_vec(groups[c], plan.group_off[c], s))) return rc;
+    p.tasks[c] = tasks[c].as<TcTask>();
+    p.ntasks[c] = static_cast<int>(plan.tasks[c].size());
+    p.groups[c] = groups[c].as<int32_t>();
+    p.ngroups[c] = static_cast<int>(plan.group_off[c].size()) - 1;
+    p.class_max_tlen[c] = plan.class_max_tlen[c];
+  }
+  p.col = col.as<int32_t>();
+  p.ncol = static_cast<uint32_t>(plan.m_plus);
+  p.partners = partners.as<TcPartner>();
+  p.queues = dg->scratch.as<unsigned>();
+  p.result = reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 64);
+  // The host vectors die with `plan`: wait for the copies.
+  MGPM_HIP(hipStreamSynchronize(s));
   TcResident &r = dg->tc;
+  r.col.swap(col);
+  r.partners.swap(partners);
+  for (int c = 0; c < kTcNumClasses; ++c) {
+    r.tasks[c].swap(tasks[c]);
+    r.groups[c].swap(groups[c]);
+  }
+  r.dev = p;
   r.params = params;
   r.on_device = false;
   r.generation++;
@@ -92,25 +121,6 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   r.m_plus = static_cast<uint64_t>(plan.m_plus);
   r.max_out = plan.max_out;
   if (!r.tuned) r.items = default_tc_items(plan.probes, plan.partners.size());
-  if ((rc = upload_vec(r.col, plan.dag_col, s))) return rc;
-  if ((rc = upload_vec(r.partners, plan.partners, s))) return rc;
-  for (int c = 0; c < kTcNumClasses; ++c) {
-    if ((rc = upload_vec(r.tasks[c], plan.tasks[c], s))) return rc;
-    if ((rc = upload_vec(r.groups[c], plan.group_off[c], s))) return rc;
-    r.dev.tasks[c] = r.tasks[c].as<TcTask>();
-    r.dev.ntasks[c] = static_cast<int>(plan.tasks[c].size());
-    r.dev.groups[c] = r.groups[c].as<int32_t>();
-    r.dev.ngroups[c] = static_cast<int>(plan.group_off[c].size()) - 1;
-    r.dev.class_max_tlen[c] = plan.class_max_tlen[c];
-  }
-  r.dev.col = r.col.as<int32_t>();
-  r.dev.ncol = static_cast<uint32_t>(plan.m_plus);
-  r.dev.partners = r.partners.as<TcPartner>();
-  r.dev.queues = dg->scratch.as<unsigned>();
-  r.dev.result =
-      reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 64);
-  // The host vectors die with `plan`: wait for the copies.
-  MGPM_HIP(hipStreamSynchronize(s));
   return MGPM_OK;
 }
 

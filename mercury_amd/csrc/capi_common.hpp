@@ -6,6 +6,7 @@
 #include <chrono>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host_graph.hpp"
@@ -56,6 +57,10 @@ struct DevBuf {
     if (e == hipSuccess) bytes = n;
     else p = nullptr;
     return e;
+  }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
   }
   template <class T>
   T *as() const {

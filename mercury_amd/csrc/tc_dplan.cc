@@ -23,11 +23,14 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
     return fail(MGPM_ERR_CAPACITY, "triangle plan: DAG has 2^32 or more arcs");
   TcResident &r = dg->tc;
   const size_t n1 = static_cast<size_t>(n) + 1;
+  // The new plan is built into these and swapped into `r` only when it is
+  // complete, so a failure leaves the resident plan untouched.
+  DevBuf new_col, new_partners, new_tasks, new_groups;
   const size_t a1 = static_cast<size_t>(nnz) + 1;
 
   // DAG into the resident column array.
   DevBuf deg, nd, src, val, pos, dag_rowptr, scan;
-  MGPM_HIP(r.col.alloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
+  MGPM_HIP(new_col.alloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
   MGPM_HIP(deg.alloc(std::max<int32_t>(n, 1) * sizeof(int32_t)));
   MGPM_HIP(nd.alloc(std::max<int32_t>(n, 1) * sizeof(int32_t)));
   MGPM_HIP(src.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
@@ -56,7 +59,7 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   d.scan_tmp = scan.p;
   d.scan_bytes = scan_bytes;
   d.out_rowptr[0] = dag_rowptr.as<int64_t>();
-  d.out_col[0] = r.col.as<int32_t>();
+  d.out_col[0] = new_col.as<int32_t>();
   if (dag_device_build(d, s)) return fail(MGPM_ERR_HIP, "DAG kernel launch failed");
 
   // Partners: the count is needed to size the rest.
@@ -80,11 +83,11 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   a.scan_bytes = scan_bytes;
   const int64_t cap = std::max<int64_t>(m, 1);  // partners <= m
   DevBuf owner, cost, cpre;
-  MGPM_HIP(r.partners.alloc(cap * sizeof(TcPartner)));
+  MGPM_HIP(new_partners.alloc(cap * sizeof(TcPartner)));
   MGPM_HIP(owner.alloc(cap * sizeof(int32_t)));
   MGPM_HIP(cost.alloc((cap + 1) * sizeof(int64_t)));
   MGPM_HIP(cpre.alloc((cap + 1) * sizeof(int64_t)));
-  a.partners = r.partners.as<TcPartner>();
+  a.partners = new_partners.as<TcPartner>();
   a.owner = owner.as<int32_t>();
   a.cost = cost.as<int64_t>();
   a.cpre = cpre.as<int64_t>();
@@ -110,14 +113,10 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   MGPM_HIP(summary.alloc(sizeof(TcDplanSummary)));
   MGPM_HIP(hipMemsetAsync(summary.p, 0, sizeof(TcDplanSummary), s));
   a.summary = summary.as<TcDplanSummary>();
-  MGPM_HIP(r.tasks[0].alloc(std::max<size_t>(nt, 1) * sizeof(TcTask)));
-  MGPM_HIP(r.groups[0].alloc((static_cast<size_t>(nt) + kTcNumClasses) * sizeof(int32_t)));
-  for (int c = 1; c < kTcNumClasses; ++c) {
-    r.tasks[c].reset();
-    r.groups[c].reset();
-  }
-  a.tasks_out = r.tasks[0].as<TcTask>();
-  a.groups_out = r.groups[0].as<int32_t>();
+  MGPM_HIP(new_tasks.alloc(std::max<size_t>(nt, 1) * sizeof(TcTask)));
+  MGPM_HIP(new_groups.alloc((static_cast<size_t>(nt) + kTcNumClasses) * sizeof(int32_t)));
+  a.tasks_out = new_tasks.as<TcTask>();
+  a.groups_out = new_groups.as<int32_t>();
   if (nt > 0) {
     MGPM_HIP(start.alloc(t1 * sizeof(int32_t)));
     MGPM_HIP(tasks_all.alloc(nt * sizeof(TcTask)));
@@ -148,7 +147,7 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
     a.gix = gix.as<int64_t>();
     if (dplan_tasks(a, np, nt, s)) return fail(MGPM_ERR_HIP, "plan kernel launch failed");
   }
-  if (dplan_dag_sums(r.col.as<int32_t>(), m, d.nd, n, dg->cus, a.summary, s))
+  if (dplan_dag_sums(new_col.as<int32_t>(), m, d.nd, n, dg->cus, a.summary, s))
     return fail(MGPM_ERR_HIP, "plan kernel launch failed");
   TcDplanSummary sum;
   int64_t cost_total = 0;
@@ -158,17 +157,9 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   MGPM_HIP(hipStreamSynchronize(s));
   if (np == 0) cost_total = 0;
 
-  r.params = params;
-  r.generation++;
-  r.s_plus = sum.s_plus;
-  r.probes = static_cast<uint64_t>(cost_total - kTcPartnerCost * np);
-  r.m_plus = static_cast<uint64_t>(m);
-  r.max_out = sum.max_out;
-  if (!r.tuned) r.items = default_tc_items(r.probes, static_cast<uint64_t>(np));
-  TcDevicePlan &p = r.dev;
-  p = TcDevicePlan();
-  const TcTask *tb = r.tasks[0].as<TcTask>();
-  const int32_t *gb = r.groups[0].as<int32_t>();
+  TcDevicePlan p;
+  const TcTask *tb = new_tasks.as<TcTask>();
+  const int32_t *gb = new_groups.as<int32_t>();
   int64_t total = 0;
   for (int c = 0; c < kTcNumClasses; ++c) {
     const int64_t cnt = nt > 0 ? sum.class_cnt[c] : 0;
@@ -184,11 +175,29 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
     p.class_max_tlen[c] = nt > 0 ? sum.class_max_tlen[c] : 0;
   }
   if (total != nt) return fail(MGPM_ERR_INTERNAL, "triangle plan: classes lost tasks");
-  p.col = r.col.as<int32_t>();
+  p.col = new_col.as<int32_t>();
   p.ncol = static_cast<uint32_t>(m);
-  p.partners = r.partners.as<TcPartner>();
+  p.partners = new_partners.as<TcPartner>();
   p.queues = dg->scratch.as<unsigned>();
   p.result = reinterpret_cast<unsigned long long *>(dg->scratch.as<char>() + 64);
+
+  // Complete: install it (the previous plan's buffers die with the locals).
+  r.col.swap(new_col);
+  r.partners.swap(new_partners);
+  r.tasks[0].swap(new_tasks);
+  r.groups[0].swap(new_groups);
+  for (int c = 1; c < kTcNumClasses; ++c) {
+    r.tasks[c].reset();
+    r.groups[c].reset();
+  }
+  r.dev = p;
+  r.params = params;
+  r.generation++;
+  r.s_plus = sum.s_plus;
+  r.probes = static_cast<uint64_t>(cost_total - kTcPartnerCost * np);
+  r.m_plus = static_cast<uint64_t>(m);
+  r.max_out = sum.max_out;
+  if (!r.tuned) r.items = default_tc_items(r.probes, static_cast<uint64_t>(np));
   r.on_device = true;
   return MGPM_OK;
 }
