@@ -35,12 +35,19 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int pattern,
   PatternPlan plan;
   build_pattern_plan(*dg->host, pattern, plan,
                      split < 0 ? UINT64_MAX : uint64_t(1) << std::min(split, 62));
+  // Upload into fresh buffers; install them only when every copy has
+  // landed, so a failure leaves the resident plan untouched.
+  DevBuf edges, edge_groups, verts, vert_groups;
   int rc;
-  if ((rc = upload(pr.edges, plan.edges, s))) return rc;
-  if ((rc = upload(pr.edge_groups, plan.edge_groups, s))) return rc;
-  if ((rc = upload(pr.verts, plan.verts, s))) return rc;
-  if ((rc = upload(pr.vert_groups, plan.vert_groups, s))) return rc;
+  if ((rc = upload(edges, plan.edges, s))) return rc;
+  if ((rc = upload(edge_groups, plan.edge_groups, s))) return rc;
+  if ((rc = upload(verts, plan.verts, s))) return rc;
+  if ((rc = upload(vert_groups, plan.vert_groups, s))) return rc;
   MGPM_HIP(hipStreamSynchronize(s));
+  pr.edges.swap(edges);
+  pr.edge_groups.swap(edge_groups);
+  pr.verts.swap(verts);
+  pr.vert_groups.swap(vert_groups);
   pr.n_edge_groups = static_cast<int>(plan.edge_groups.size()) - 1;
   pr.n_vert_groups = static_cast<int>(plan.vert_groups.size()) - 1;
   pr.edge_group_cost = std::move(plan.edge_group_cost);
