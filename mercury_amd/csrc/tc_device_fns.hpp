@@ -18,12 +18,14 @@ constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
 // items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Batches
 // whose lists average `cursor_min` items or more (never if <= 0) take the
 // owner-cursor path.  Returns this lane's hit count.
+// `pre` (if non-null) is this lane's record of the first batch, already
+// loaded by the caller (so its load overlapped the table build).
 template <int ITEMS, class Probe>
 __device__ __forceinline__ uint32_t stream_partners(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
     int first_batch, int batch_stride, int32_t *mark_lds,
-    const Probe &probe, int cursor_min) {
+    const Probe &probe, int cursor_min, const TcPartner *pre = nullptr) {
   // The start marks are written by one lane and read by another.  They are
   // LDS accesses (ds_*), which a wave executes in order; lds_order() keeps
   // the compiler from reordering or forwarding them across each handoff.
@@ -34,7 +36,10 @@ __device__ __forceinline__ uint32_t stream_partners(
     const int pi = pb + lane;
     uint32_t beg = 0;
     int len = 0;
-    if (pi < pcnt) {
+    if (pre && pb == first_batch * 64) {
+      beg = pre->beg;
+      len = static_cast<int>(pre->len);
+    } else if (pi < pcnt) {
       const TcPartner pr = partners[pbeg + pi];
       beg = pr.beg;
       len = static_cast<int>(pr.len);
@@ -259,6 +264,10 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
     return stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0,
                                   1, mark, probe, cursor_min);
   }
+  // This lane's partner record of the first batch, loaded before the table
+  // is built so the two loads overlap.
+  TcPartner pre{0u, 0u};
+  if (lane < task.pcnt) pre = partners[task.pbeg + lane];
   // 8 slots per key where the table fits (load <= 1/8): fewer probe rounds.
   int logp = ceil_log2(8 * task.tlen);
   logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
@@ -273,13 +282,14 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
     HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
     const uint32_t hits = stream_partners<ITEMS>(
         col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe,
-        cursor_min);
+        cursor_min, &pre);
     wave_sync();
     return hits;
   }
   HashProbe<true> probe{table, logp, static_cast<uint32_t>(P - 1)};
   const uint32_t hits = stream_partners<ITEMS>(
-      col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min);
+      col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min,
+      &pre);
   wave_sync();
   return hits;
 }
