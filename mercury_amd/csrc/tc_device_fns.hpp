@@ -12,126 +12,145 @@ namespace tcdev {
 
 constexpr int kMaxItems = 8;  // list items per lane per step (4 or 8)
 
-// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`.
-// Partners are taken 64 at a time (batches first_batch, first_batch +
-// batch_stride, ...); their lists are concatenated and walked 64*ITEMS
-// items per step.  `mark` is 64*ITEMS ints of wave-private LDS.  Batches
-// whose lists average `cursor_min` items or more (never if <= 0) take the
-// owner-cursor path.  Returns this lane's hit count.
-// `pre` (if non-null) is this lane's record of the first batch, already
-// loaded by the caller (so its load overlapped the table build).
+// Streams one batch of up to 64 partner lists through `probe`: this lane's
+// partner list starts at col[beg] and holds `len` items (0 for lanes past
+// the batch).  The lists are concatenated and walked 64*ITEMS items per
+// step.  `mark` is 64*ITEMS ints of wave-private LDS.  A batch whose lists
+// average `cursor_min` items or more (never if <= 0) takes the owner-cursor
+// path.  Returns this lane's hit count.
+template <int ITEMS, class Probe>
+__device__ __forceinline__ uint32_t stream_batch(
+    const int32_t *__restrict__ col, uint32_t ncol, uint32_t beg, int len,
+    lds_i32 *mark, const Probe &probe, int cursor_min, int lane) {
+  // The start marks are written by one lane and read by another.  They are
+  // LDS accesses (ds_*), which a wave executes in order; lds_order() keeps
+  // the compiler from reordering or forwarding them across each handoff.
+  uint32_t hits = 0;
+  const int incl = wave_incl_scan(len);
+  const int excl = incl - len;
+  const int total = __builtin_amdgcn_readlane(incl, 63);
+  if (cursor_min > 0 && total >= cursor_min * 64) {
+    // Long lists on average: each lane walks its own owner cursor over
+    // the batch's prefix sums (kept in LDS: a cursor advances in
+    // divergent code, where cross-lane shuffles would read inactive
+    // lanes).  Items lane + 64 i advance the cursor by at most a few
+    // partners per step.
+    lds_order();  // previous reads of the marks are done
+    mark[lane] = incl;
+    mark[64 + lane] = static_cast<int32_t>(beg);
+    lds_order();
+    int o = 0, oexcl = 0, onext = mark[0];
+    uint32_t obeg = static_cast<uint32_t>(mark[64]);
+    for (int k = 0; k < total; k += 64 * ITEMS) {
+      int32_t w[ITEMS];
+      bool ok[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const int item = k + 64 * i + lane;
+        ok[i] = item < total;
+        if (ok[i]) {
+          while (item >= onext) {
+            ++o;
+            oexcl = onext;
+            onext = mark[o];
+            obeg = static_cast<uint32_t>(mark[64 + o]);
+          }
+        }
+        const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
+        ok[i] = ok[i] && idx < ncol;
+        w[i] = ok[i] ? col[idx] : -2;
+      }
+      hits += probe.template lookup<ITEMS>(w, ok);
+    }
+    return hits;
+  }
+  for (int k = 0; k < total; k += (64 * ITEMS)) {
+    lds_order();  // previous reads of the marks are done
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;
+    lds_order();
+    if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
+    lds_order();
+    unsigned long long starts[ITEMS];
+    int own[ITEMS];  // a partner starts at this lane's item i
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const bool st = mark[lane + 64 * i] != 0;
+      starts[i] = __ballot(st);
+      own[i] = st ? 1 : 0;
+    }
+    // The partner owning item g is the last one with excl <= g; item g
+    // of the batch is col[g + (beg - excl)] of its owner.  Every partner
+    // list lies inside the DAG column array (checked when the plan is
+    // built), so only items past the batch's end need a guard.
+    int base = __popcll(__ballot(excl <= k)) - 1;
+    const int delta = static_cast<int>(beg - static_cast<uint32_t>(excl));
+    int32_t w[ITEMS];
+    bool ok[ITEMS];
+    if (k + 64 * ITEMS <= total) {  // a full step: no guards
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const int owner = base + bits_below(starts[i]) + own[i];
+        base += __popcll(starts[i]);
+        const uint32_t odelta = static_cast<uint32_t>(
+            __builtin_amdgcn_ds_bpermute(owner << 2, delta));
+        w[i] = col[static_cast<uint32_t>(k + 64 * i + lane) + odelta];
+        ok[i] = true;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const int owner = base + bits_below(starts[i]) + own[i];
+        base += __popcll(starts[i]);
+        const uint32_t odelta = static_cast<uint32_t>(
+            __builtin_amdgcn_ds_bpermute(owner << 2, delta));
+        const int item = k + 64 * i + lane;
+        ok[i] = item < total;
+        // Unconditional gather (idle items read col[0]): no branch.
+        const int32_t v =
+            col[ok[i] ? static_cast<uint32_t>(item) + odelta : 0u];
+        w[i] = ok[i] ? v : -2;
+      }
+    }
+    hits += probe.template lookup<ITEMS>(w, ok);
+  }
+  return hits;
+}
+
+// Streams the partner lists [pbeg, pbeg+pcnt) of one task through `probe`,
+// 64 partners at a time (batches first_batch, first_batch + batch_stride,
+// ...).  `pre` (if non-null) is this lane's record of the first batch
+// (valid for lanes below pcnt), already loaded by the caller; that batch
+// is streamed outside the loop, so no join waits on loads issued after
+// `pre`.  Returns this lane's hit count.
 template <int ITEMS, class Probe>
 __device__ __forceinline__ uint32_t stream_partners(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcPartner *__restrict__ partners, uint32_t pbeg, int pcnt,
     int first_batch, int batch_stride, int32_t *mark_lds,
     const Probe &probe, int cursor_min, const TcPartner *pre = nullptr) {
-  // The start marks are written by one lane and read by another.  They are
-  // LDS accesses (ds_*), which a wave executes in order; lds_order() keeps
-  // the compiler from reordering or forwarding them across each handoff.
   lds_i32 *mark = as_lds(mark_lds);
   const int lane = lane64();
   uint32_t hits = 0;
-  for (int pb = first_batch * 64; pb < pcnt; pb += batch_stride * 64) {
+  int pb = first_batch * 64;
+  if (pre && pb < pcnt) {
+    const bool mine = pb + lane < pcnt;
+    hits += stream_batch<ITEMS>(col, ncol, mine ? pre->beg : 0u,
+                                mine ? static_cast<int>(pre->len) : 0, mark,
+                                probe, cursor_min, lane);
+    pb += batch_stride * 64;
+  }
+  for (; pb < pcnt; pb += batch_stride * 64) {
     const int pi = pb + lane;
     uint32_t beg = 0;
     int len = 0;
-    if (pre && pb == first_batch * 64) {
-      beg = pre->beg;
-      len = static_cast<int>(pre->len);
-    } else if (pi < pcnt) {
+    if (pi < pcnt) {
       const TcPartner pr = partners[pbeg + pi];
       beg = pr.beg;
       len = static_cast<int>(pr.len);
     }
-    const int incl = wave_incl_scan(len);
-    const int excl = incl - len;
-    const int total = __builtin_amdgcn_readlane(incl, 63);
-    if (cursor_min > 0 && total >= cursor_min * 64) {
-      // Long lists on average: each lane walks its own owner cursor over
-      // the batch's prefix sums (kept in LDS: a cursor advances in
-      // divergent code, where cross-lane shuffles would read inactive
-      // lanes).  Items lane + 64 i advance the cursor by at most a few
-      // partners per step.
-      lds_order();  // previous reads of the marks are done
-      mark[lane] = incl;
-      mark[64 + lane] = static_cast<int32_t>(beg);
-      lds_order();
-      int o = 0, oexcl = 0, onext = mark[0];
-      uint32_t obeg = static_cast<uint32_t>(mark[64]);
-      for (int k = 0; k < total; k += 64 * ITEMS) {
-        int32_t w[ITEMS];
-        bool ok[ITEMS];
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-          const int item = k + 64 * i + lane;
-          ok[i] = item < total;
-          if (ok[i]) {
-            while (item >= onext) {
-              ++o;
-              oexcl = onext;
-              onext = mark[o];
-              obeg = static_cast<uint32_t>(mark[64 + o]);
-            }
-          }
-          const uint32_t idx = obeg + static_cast<uint32_t>(item - oexcl);
-          ok[i] = ok[i] && idx < ncol;
-          w[i] = ok[i] ? col[idx] : -2;
-        }
-        hits += probe.template lookup<ITEMS>(w, ok);
-      }
-      continue;
-    }
-    for (int k = 0; k < total; k += (64 * ITEMS)) {
-      lds_order();  // previous reads of the marks are done
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) mark[lane + 64 * i] = 0;
-      lds_order();
-      if (excl > k && excl < k + (64 * ITEMS)) mark[excl - k] = 1;
-      lds_order();
-      unsigned long long starts[ITEMS];
-      int own[ITEMS];  // a partner starts at this lane's item i
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
-        const bool st = mark[lane + 64 * i] != 0;
-        starts[i] = __ballot(st);
-        own[i] = st ? 1 : 0;
-      }
-      // The partner owning item g is the last one with excl <= g; item g
-      // of the batch is col[g + (beg - excl)] of its owner.  Every partner
-      // list lies inside the DAG column array (checked when the plan is
-      // built), so only items past the batch's end need a guard.
-      int base = __popcll(__ballot(excl <= k)) - 1;
-      const int delta = static_cast<int>(beg - static_cast<uint32_t>(excl));
-      int32_t w[ITEMS];
-      bool ok[ITEMS];
-      if (k + 64 * ITEMS <= total) {  // a full step: no guards
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-          const int owner = base + bits_below(starts[i]) + own[i];
-          base += __popcll(starts[i]);
-          const uint32_t odelta = static_cast<uint32_t>(
-              __builtin_amdgcn_ds_bpermute(owner << 2, delta));
-          w[i] = col[static_cast<uint32_t>(k + 64 * i + lane) + odelta];
-          ok[i] = true;
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-          const int owner = base + bits_below(starts[i]) + own[i];
-          base += __popcll(starts[i]);
-          const uint32_t odelta = static_cast<uint32_t>(
-              __builtin_amdgcn_ds_bpermute(owner << 2, delta));
-          const int item = k + 64 * i + lane;
-          ok[i] = item < total;
-          // Unconditional gather (idle items read col[0]): no branch.
-          const int32_t v =
-              col[ok[i] ? static_cast<uint32_t>(item) + odelta : 0u];
-          w[i] = ok[i] ? v : -2;
-        }
-      }
-      hits += probe.template lookup<ITEMS>(w, ok);
-    }
+    hits += stream_batch<ITEMS>(col, ncol, beg, len, mark, probe, cursor_min,
+                                lane);
   }
   return hits;
 }
@@ -248,48 +267,95 @@ __device__ __forceinline__ void flush_hits(unsigned long long acc,
 }
 
 
+// What a wave loads for a task before building its table: this lane's
+// partner record of the first batch and this lane's owner items (lane,
+// lane + 64; used for owners of up to 128 items).  The loads are issued
+// unconditionally at clamped (valid) addresses and their values are
+// selected only where they are used, so loads issued one task ahead stay
+// in flight across the previous task's work (a select right after a load
+// would wait for it).
+struct TaskPre {
+  TcPartner first{0u, 0u};
+  int32_t key0 = 0, key1 = 0;
+};
+
+constexpr int kTaskPreMaxTlen = 128;
+
+// Tasks have pcnt >= 1 and tlen >= 1 (owners without partners get none).
+__device__ __forceinline__ TaskPre load_task_pre(
+    const int32_t *__restrict__ col, const TcPartner *__restrict__ partners,
+    const TcTask &t, int lane) {
+  TaskPre p;
+  p.first = partners[t.pbeg + static_cast<uint32_t>(min(lane, t.pcnt - 1))];
+  p.key0 = col[t.tbeg + static_cast<uint32_t>(min(lane, t.tlen - 1))];
+  p.key1 = col[t.tbeg + static_cast<uint32_t>(min(lane + 64, t.tlen - 1))];
+  return p;
+}
+
+// table_insert of up to two keys per lane already in registers (valid
+// flags v0, v1).
+__device__ __forceinline__ void table_insert_keys(int32_t *table, int logp,
+                                                  int32_t k0, bool v0,
+                                                  int32_t k1, bool v1,
+                                                  bool quad) {
+  const uint32_t mask = (1u << logp) - 1u;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int32_t key = k == 0 ? k0 : k1;
+    if (!(k == 0 ? v0 : v1)) continue;
+    uint32_t h = slot_of(key, logp);
+    uint32_t step = 1;
+    while (atomicCAS(&table[h], -1, key) != -1) {
+      h = (h + (quad ? step : 1u)) & mask;
+      ++step;
+    }
+  }
+}
+
 // Counts one task: builds the owner's hash set in the wave-private table
 // (2^logp_max slots, all lanes of this wave) and streams its partners.  A
 // table that would not fit falls back to binary search in global memory.
 template <int ITEMS>
+// `pre` holds what load_task_pre loaded for this task.
 __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
                                               uint32_t ncol,
                                               const TcPartner *__restrict__ partners,
                                               const TcTask &task, int32_t *table,
                                               int logp_max, int32_t *mark,
-                                              int cursor_min) {
+                                              int cursor_min,
+                                              const TaskPre &pre) {
   const int lane = lane64();
   if (2 * task.tlen > (1 << logp_max)) {
     SortedProbe probe{col + task.tbeg, task.tlen};
     return stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0,
-                                  1, mark, probe, cursor_min);
+                                  1, mark, probe, cursor_min, &pre.first);
   }
-  // This lane's partner record of the first batch, loaded before the table
-  // is built so the two loads overlap.
-  TcPartner pre{0u, 0u};
-  if (lane < task.pcnt) pre = partners[task.pbeg + lane];
   // 8 slots per key where the table fits (load <= 1/8): fewer probe rounds.
   int logp = ceil_log2(8 * task.tlen);
   logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
   const int P = 1 << logp;
   for (int i = lane; i < P; i += 64) table[i] = -1;
   wave_sync();
-  table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64,
-               logp_max > 8);
+  if (task.tlen <= kTaskPreMaxTlen)
+    table_insert_keys(table, logp, pre.key0, lane < task.tlen, pre.key1,
+                      lane + 64 < task.tlen, logp_max > 8);
+  else
+    table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64,
+                 logp_max > 8);
   wave_sync();
   // Tables of up to 256 slots (small owners) probe item by item.
   if (logp_max <= 8) {
     HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
     const uint32_t hits = stream_partners<ITEMS>(
         col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe,
-        cursor_min, &pre);
+        cursor_min, &pre.first);
     wave_sync();
     return hits;
   }
   HashProbe<true> probe{table, logp, static_cast<uint32_t>(P - 1)};
   const uint32_t hits = stream_partners<ITEMS>(
       col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min,
-      &pre);
+      &pre.first);
   wave_sync();
   return hits;
 }

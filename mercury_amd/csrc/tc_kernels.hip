@@ -50,9 +50,19 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     g = wave_first(g);
     if (g >= static_cast<uint32_t>(ngroups)) break;
     const int t_end = goff[g + 1];
-    for (int t = goff[g]; t < t_end; ++t)
-      acc += wave_task<ITEMS>(col, ncol, partners, tasks[t], table, LOGP_MAX,
-                              s_mark[wv], cursor_min);
+    int t = goff[g];
+    TcTask task = tasks[t];
+    TaskPre pre = load_task_pre(col, partners, task, lane);
+    for (; t < t_end; ++t) {
+      // The next task's loads go out before this task's gathers (the last
+      // task of the group reloads itself: same load count on every path).
+      const TcTask next = tasks[t + 1 < t_end ? t + 1 : t];
+      const TaskPre next_pre = load_task_pre(col, partners, next, lane);
+      acc += wave_task<ITEMS>(col, ncol, partners, task, table, LOGP_MAX,
+                              s_mark[wv], cursor_min, pre);
+      task = next;
+      pre = next_pre;
+    }
   }
   flush_hits(acc, result);
 }
