@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker / CPU baseline.  The product
+bench.py's cpu_baseline leg as the parity reference / CPU baseline.  The product
 never imports it.  Build with ``make -C oracle`` (or __graft_entry__.build()).
 """
 from __future__ import annotations

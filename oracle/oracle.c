@@ -1,7 +1,7 @@
 /*
  * oracle.c — CPU restatement of the reference's counting path.
  *
- * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker and the CPU
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the parity reference and the CPU
  * baseline, never the product.  Set algebra is done by sorted merges, so
  * it shares no machinery with the GPU kernels (hash tables, binary search).
  * All counts are exact uint64.

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
  * bench.py's cpu_baseline leg may load this library, and only as the
- * checker / CPU baseline; the product (mercury_amd) never calls it.
+ * parity reference / CPU baseline; the product (mercury_amd) never calls it.
  *
  * All graphs are simple undirected CSR (sorted rows, no self-loops, no
  * duplicates, symmetric): rowptr int64[n+1], colidx int32[rowptr[n]].
