@@ -145,9 +145,9 @@ def bench_batch(ma, args) -> int:
     sync = lambda: ma.native.check(ma.native.lib.mgpm_device_synchronize(0),
                                    "sync")
     # one launch (and one synchronisation) per query, with the kernels the
-    # batch runs (P1/P4 by their per-edge kernels, not the census)
+    # batch runs (P1, P4, P7, P8 by their per-edge kernels, not the census)
     kern = ma.make_tune(pattern_census=-1)
-    one = lambda dg, p: dg.pattern_count(p, kern if p in (1, 4) else None)[0]
+    one = lambda dg, p: dg.pattern_count(p, kern if p in (1, 4, 7, 8) else None)[0]
     for _ in range(args.warmup):
         single = [one(dg, p) for dg, p in queries]
     sync()
@@ -237,7 +237,7 @@ def bench_dag(ma, args) -> int:
 def bench_patterns(ma, args) -> int:
     """P1..P8 counts on one RMAT graph (default RMAT-20, ef 16, seed 4, the
     pattern configuration): device time per count for each requested
-    pattern and method ("census": P1/P4 closed forms; "kernels": the
+    pattern and method ("census": P1, P4, P7, P8 closed forms; "kernels": the
     reference's per-edge set algebra), and whether the methods agree."""
     scale, ef, seed = args.pattern_graph
     names = [p.strip() for p in args.patterns.split(",")]
@@ -251,7 +251,7 @@ def bench_patterns(ma, args) -> int:
         p = ma.PATTERN_NAMES[name]
         counts = {}
         for meth in methods:
-            if meth == "census" and p not in (1, 4):
+            if meth == "census" and p not in (1, 4, 7, 8):
                 continue
             tune = ma.make_tune(pattern_census=-1) if meth == "kernels" else None
             dg.pattern_count(p, tune)  # builds plans outside the timing
@@ -265,7 +265,7 @@ def bench_patterns(ma, args) -> int:
             rows[f"{name}_{meth}_ms"] = sorted(ks)[len(ks) // 2]
             rows[f"{name}_count"] = c
         agree = agree and len(set(counts.values())) <= 1
-    first = f"{names[0]}_{methods[0] if names[0] in ('P1', 'P4') else 'kernels'}_ms"
+    first = f"{names[0]}_{methods[0] if names[0] in ('P1', 'P4', 'P7', 'P8') else 'kernels'}_ms"
     out = {
         "metric": "pattern count time", "value": rows.get(first),
         "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,

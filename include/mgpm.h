@@ -96,9 +96,10 @@ typedef struct mgpm_tune {
                              estimated pair tests exceed 2^pattern_split is
                              cut into shares run by different waves
                              (default 20; -1 = never) */
-  int32_t pattern_census; /* P1, P4: 0 (default) or 1 = closed form from
-                             a triangle and 4-cycle census; -1 = the
-                             reference's per-edge set algebra */
+  int32_t pattern_census; /* P1, P4, P7, P8: 0 (default) or 1 = closed
+                             form from a triangle, 4-cycle, 4-clique and
+                             5-clique census; -1 = the reference's per-edge
+                             set algebra */
 } mgpm_tune;
 
 /* Per-call measurements. */

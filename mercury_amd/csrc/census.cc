@@ -1,5 +1,6 @@
-// P1 (induced claws) and P4 (induced paths on 4 vertices) in closed form
-// from a triangle and 4-cycle census on the device.
+// P1 (induced claws), P4 (induced paths on 4 vertices), P7 (induced
+// K1,1,3) and P8 (induced K4 plus a pendant edge) in closed form from a
+// triangle, 4-cycle, 4-clique and 5-clique census on the device.
 //
 // The reference counts P1 per directed edge with frequency arrays
 // (pattern_kernels.cuh:6-133, combined as sm_multigpu.cu:209-232); its sum
@@ -22,6 +23,24 @@
 // diamonds (1) and K4s (3), with diamonds = sum C(t_e,2) - 6 K4:
 //   P4 = sum_edges (d_x - t - 1)(d_y - t - 1)
 //        - 4 (C4 - sum_edges C(t_e,2) + 3 K4).
+//
+// P7 (pattern_kernels.cuh:352-459) is twice the induced K1,1,3: per edge
+// e, the independent triples of its common neighbourhood C_e (|C_e| =
+// t_e).  Inclusion-exclusion over the three possible edges of a triple
+// gives C(t,3) - (t-2) E(C_e) + sum_{c in C_e} C(d_C(c), 2) - T(C_e),
+// where E(C_e) = k4_e (4-cliques on e), d_C(c) = k_T (4-cliques on the
+// triangle T = e + c) and T(C_e) = 5-cliques on e.  Summed over edges, and
+// with sum_e (t_e - 2) k4_e = 1/2 sum_T k_T (t_ab + t_ac + t_bc - 6):
+//   P7 = 2 (sum_e C(t_e,3) - 1/2 sum_T k_T (sum_{e in T} t_e - 6)
+//           + 3 sum_T C(k_T,2) - 10 K5).
+// P8 (pattern_kernels.cuh:461-572) is three times the induced K4 plus a
+// pendant edge: per 4-clique Q and a in Q, the neighbours of a outside Q
+// adjacent to no other vertex of Q, by inclusion-exclusion over those
+// three: (d_a - 3) - sum_b (t_ab - 2) + sum_{b,c} (k_abc - 1) - K5(Q).
+// Summed over Q and a, with sum_v (d_v - 3) k4_v = 1/3 sum_T k_T
+// (d_a + d_b + d_c - 9):
+//   P8 = 3 (1/3 sum_T k_T (d_a + d_b + d_c - 9)
+//           - sum_T k_T (sum_{e in T} t_e - 6) + 6 sum_T C(k_T,2) - 20 K5).
 // Exact in uint64 (mod 2^64 in the intermediate terms, like the
 // reference's unsigned counters).
 #include <algorithm>
@@ -33,13 +52,18 @@
 
 namespace mgpm {
 
-// Device bytes of the 4-cycle pass's per-workgroup arrays.
+// Device bytes of the 4-cycle pass's per-workgroup arrays, and of the
+// 5-clique pass's per-wave slabs.
 constexpr size_t kCensusC4Budget = size_t(4) << 30;
+constexpr size_t kCensusK5Budget = size_t(8) << 30;
 
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms) {
-  if (pattern != MGPM_P1 && pattern != MGPM_P4)
-    return fail(MGPM_ERR_PATTERN, "census: only P1 and P4 have closed forms");
+  if (pattern != MGPM_P1 && pattern != MGPM_P4 && pattern != MGPM_P7 &&
+      pattern != MGPM_P8)
+    return fail(MGPM_ERR_PATTERN,
+                "census: only P1, P4, P7 and P8 have closed forms");
+  const bool k5 = pattern == MGPM_P7 || pattern == MGPM_P8;
   const CsrGraph &G = *dg->host;
   const int64_t nnz = G.nnz;
   const int32_t n = G.n;
@@ -77,6 +101,25 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
     MGPM_HIP(touched.alloc(groups * per));
   }
 
+  // P7 and P8 also take the 4-cliques on every triangle and the
+  // 5-cliques: per wave a slab sized by the largest DAG out-degree and the
+  // largest degree.
+  DevBuf k5_slab;
+  int32_t max_deg = 1;
+  int64_t k5_len = 0;
+  int k5_waves = 0;
+  if (k5) {
+    for (int32_t v = 0; v < n; ++v)
+      max_deg = std::max<int32_t>(
+          max_deg, static_cast<int32_t>(G.rowptr[v + 1] - G.rowptr[v]));
+    k5_len = census_k5_slab_len(slab_len, max_deg);
+    const size_t per = static_cast<size_t>(k5_len) * sizeof(int32_t);
+    const size_t fit = kCensusK5Budget / per;
+    k5_waves = static_cast<int>(
+        std::max<size_t>(4, std::min<size_t>(waves, fit)) & ~size_t(3));
+    MGPM_HIP(k5_slab.alloc(static_cast<size_t>(k5_waves) * per));
+  }
+
   DagDevice d;
   d.rowptr = dg->csr.rowptr.as<int64_t>();
   d.col = dg->csr.colidx.as<int32_t>();
@@ -111,6 +154,10 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
   c.wedges = wedges.as<uint32_t>();
   c.touched = touched.as<int32_t>();
   c.c4_groups = groups;
+  c.k5_slab = k5_slab.as<int32_t>();
+  c.k5_len = k5_len;
+  c.max_deg = max_deg;
+  c.k5_waves = k5_waves;
 
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (dag_device_build(d, s)) return fail(MGPM_ERR_HIP, "DAG kernel launch failed");
@@ -123,17 +170,32 @@ int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
     if (census_c4_launch(c, s))
       return fail(MGPM_ERR_HIP, "census kernel launch failed");
   }
+  if (k5 && census_k5_launch(c, s))
+    return fail(MGPM_ERR_HIP, "census kernel launch failed");
   MGPM_HIP(hipEventRecord(dg->ev1, s));
   unsigned long long acc[kCensusSlots];
   MGPM_HIP(hipMemcpyAsync(acc, c.acc, sizeof(acc), hipMemcpyDeviceToHost, s));
   MGPM_HIP(hipStreamSynchronize(s));
   MGPM_HIP(hipEventElapsedTime(kernel_ms, dg->ev0, dg->ev1));
-  if (pattern == MGPM_P1)
+  if (pattern == MGPM_P1) {
     *count = acc[kCensusDeg3] - acc[kCensusTriDegrees] +
              2 * acc[kCensusEdgePairs] - 4 * acc[kCensusK4];
-  else  // induced 4-cycles = C4 - sum C(t_e,2) + 3 K4
+  } else if (pattern == MGPM_P4) {  // induced 4-cycles = C4 - sum C(t_e,2) + 3 K4
     *count = acc[kCensusPathPairs] -
              4 * (acc[kCensusC4] - acc[kCensusEdgePairs] + 3 * acc[kCensusK4]);
+  } else {
+    // Two independent counts of the 4-cliques, every common neighbourhood
+    // of the size its triangle count says, and the halving / third exact.
+    if (acc[kCensusMismatch] != 0 || acc[kCensusK4Rows] != acc[kCensusK4] ||
+        acc[kCensusTriEdgeK] % 2 != 0 || acc[kCensusTriDegK] % 3 != 0)
+      return fail(MGPM_ERR_INTERNAL, "census: 4-clique terms disagree");
+    const uint64_t tek = acc[kCensusTriEdgeK], kk = acc[kCensusTriPairs],
+                   k5c = acc[kCensusK5];
+    if (pattern == MGPM_P7)
+      *count = 2 * (acc[kCensusEdgeTriples] - tek / 2 + 3 * kk - 10 * k5c);
+    else
+      *count = 3 * (acc[kCensusTriDegK] / 3 - tek + 6 * kk - 20 * k5c);
+  }
   return MGPM_OK;
 }
 

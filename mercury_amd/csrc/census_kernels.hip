@@ -12,7 +12,9 @@
 //   * K4s through the triangle set of the arc: ordered pairs (z, q) of C
 //     with q ∈ N+(z) -- each K4 {x, y, z, q} once -- by pairs for small C,
 //     by per-z intersections for large C.
-// A second pass over the DAG arcs sums C(t_e, 2) and C(d_v, 3).
+// A second pass over the DAG arcs sums C(t_e, 2), C(t_e, 3) and C(d_v, 3).
+// A third pass (census_k5, P7 and P8 only) takes the 4-cliques on every
+// triangle and the 5-cliques.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
 // Sums over vertices (C(d,3)) and over DAG arcs (C(t,2) and
 // (d_x - t - 1)(d_y - t - 1)).
 __global__ void census_sums(CensusArgs c) {
-  unsigned long long s3 = 0, s2 = 0, sp = 0;
+  unsigned long long s3 = 0, s2 = 0, sp = 0, st = 0;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < c.n;
        v += int64_t(gridDim.x) * blockDim.x) {
     const unsigned long long d = static_cast<unsigned long long>(c.deg[v]);
@@ -154,16 +156,217 @@ __global__ void census_sums(CensusArgs c) {
       const unsigned long long dy =
           static_cast<unsigned long long>(c.deg[c.dcol[a]]);
       s2 += t * (t - 1) / 2;
+      st += t >= 3 ? t * (t - 1) * (t - 2) / 6 : 0;
       sp += (d - t - 1) * (dy - t - 1);
     }
   }
   s3 = wave_sum_u64(s3);
   s2 = wave_sum_u64(s2);
   sp = wave_sum_u64(sp);
+  st = wave_sum_u64(st);
   if (lane64() == 0) {
     if (s3) atomicAdd(&c.acc[kCensusDeg3], s3);
     if (s2) atomicAdd(&c.acc[kCensusEdgePairs], s2);
     if (sp) atomicAdd(&c.acc[kCensusPathPairs], sp);
+    if (st) atomicAdd(&c.acc[kCensusEdgeTriples], st);
+  }
+}
+
+// Per-triangle 4-clique terms and 5-cliques, one wave per DAG arc x->y
+// (arcs handed out by an atomic cursor).  C = N+(x) ∩ N+(y) is the set of
+// triangles {x, y, z} seen at this arc (each triangle once), kept with the
+// positions of x->z and y->z in the DAG column array.
+//   * k_T = |N(x) ∩ N(y) ∩ N(z)|, the 4-cliques on the triangle, from the
+//     full common neighbourhood F = N(x) ∩ N(y) (|F| must equal t_e);
+//     the triangle adds C(k_T, 2), k_T (t_xy + t_xz + t_yz - 6) and
+//     k_T (d_x + d_y + d_z - 9);
+//   * the DAG adjacency inside C as bit rows (bit j of row i: C[j] in
+//     N+(C[i])): row i's bits are the 4-cliques {x, y, C[i], C[j]}, and
+//     |row i ∩ row j| over the bits j of row i are the 5-cliques, each
+//     once (x, y, C[i], C[j], C[k] in DAG order).
+// Slab per wave: C, the two position arrays and the row-bit lists (max
+// out-degree each), F (max degree), then the rows (max_out x
+// ceil(max_out / 64) words).
+constexpr int kSerialKt = 128;  // |F| up to this: one triangle per lane
+
+__global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
+  const int lane = lane64();
+  const int64_t wid = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  const int64_t L = c.slab_len;
+  int32_t *Cz = c.k5_slab + wid * c.k5_len;
+  uint32_t *Cxz = reinterpret_cast<uint32_t *>(Cz + L);
+  uint32_t *Cyz = reinterpret_cast<uint32_t *>(Cz + 2 * L);
+  int32_t *ML = Cz + 3 * L;
+  int32_t *F = Cz + 4 * L;
+  unsigned long long *rows = reinterpret_cast<unsigned long long *>(
+      Cz + ((4 * L + c.max_deg + 1) & ~int64_t(1)));
+  unsigned long long kk = 0, tek = 0, tdk = 0, k4 = 0, k5 = 0, bad = 0;
+  for (;;) {
+    uint32_t a = 0;
+    if (lane == 0) a = atomicAdd(c.cursor + 2, 1u);
+    a = wave_first(a);
+    if (static_cast<int64_t>(a) >= c.m) break;
+    const uint32_t ta = c.t[a];
+    if (ta == 0) continue;  // no triangle on the edge
+    const int32_t x = c.dsrc[a], y = c.dcol[a];
+    const int64_t bx = c.drp[x], by = c.drp[y];
+    const int lx = static_cast<int>(c.drp[x + 1] - bx);
+    const int ly = static_cast<int>(c.drp[y + 1] - by);
+    const bool xs = lx <= ly;
+    const int64_t sb = xs ? bx : by, lb = xs ? by : bx;
+    const int ls = xs ? lx : ly, ll = xs ? ly : lx;
+    const int32_t *S = c.dcol + sb;
+    const int32_t *Lg = c.dcol + lb;
+    int cnt = 0;
+    for (int p0 = 0; p0 < ls; p0 += 64) {
+      const int p = p0 + lane;
+      bool hit = false;
+      int32_t z = 0;
+      int j = 0;
+      if (p < ls) {
+        z = S[p];
+        j = lower_bound_g(Lg, ll, z);
+        hit = j < ll && Lg[j] == z;
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (hit) {
+        const int o = cnt + bits_below(mask);
+        const uint32_t is = static_cast<uint32_t>(sb + p);
+        const uint32_t il = static_cast<uint32_t>(lb + j);
+        Cz[o] = z;
+        Cxz[o] = xs ? is : il;
+        Cyz[o] = xs ? il : is;
+      }
+      cnt += __popcll(mask);
+    }
+    if (cnt == 0) continue;  // x and y are not the lowest of any triangle
+    // F = N(x) ∩ N(y) over the undirected lists.
+    const int dx = c.deg[x], dy = c.deg[y];
+    const bool wx = dx <= dy;
+    const int32_t *W = c.col + c.rowptr[wx ? x : y];
+    const int32_t *Sr = c.col + c.rowptr[wx ? y : x];
+    const int nw = wx ? dx : dy, ns = wx ? dy : dx;
+    int te = 0;
+    for (int p0 = 0; p0 < nw; p0 += 64) {
+      const int p = p0 + lane;
+      bool hit = false;
+      int32_t w = 0;
+      if (p < nw) {
+        w = W[p];
+        hit = contains_g(Sr, ns, w);
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (hit) F[te + bits_below(mask)] = w;
+      te += __popcll(mask);
+    }
+    if (lane == 0 && static_cast<uint32_t>(te) != ta) ++bad;
+    wave_sync();  // C and F written by other lanes
+    const unsigned long long dxy = static_cast<unsigned long long>(dx + dy);
+    if (te <= kSerialKt) {
+      for (int i = lane; i < cnt; i += 64) {
+        const int32_t z = Cz[i];
+        const int dz = c.deg[z];
+        const int32_t *Nz = c.col + c.rowptr[z];
+        unsigned k = 0;
+        if (te <= dz) {
+          for (int q = 0; q < te; ++q) k += contains_g(Nz, dz, F[q]);
+        } else {
+          for (int q = 0; q < dz; ++q) k += contains_g(F, te, Nz[q]);
+        }
+        if (k) {
+          const unsigned long long kt = k;
+          kk += kt * (kt - 1) / 2;
+          tek += kt * (static_cast<unsigned long long>(ta) + c.t[Cxz[i]] +
+                       c.t[Cyz[i]] - 6);
+          tdk += kt * (dxy + static_cast<unsigned long long>(dz) - 9);
+        }
+      }
+    } else {
+      for (int i = 0; i < cnt; ++i) {
+        const int32_t z = Cz[i];
+        const int dz = c.deg[z];
+        const int32_t *Nz = c.col + c.rowptr[z];
+        int part = 0;
+        if (te <= dz) {
+          for (int q = lane; q < te; q += 64) part += contains_g(Nz, dz, F[q]);
+        } else {
+          for (int q = lane; q < dz; q += 64) part += contains_g(F, te, Nz[q]);
+        }
+        const unsigned k = static_cast<unsigned>(wave_sum_i32(part));
+        if (lane == 0 && k) {
+          const unsigned long long kt = k;
+          kk += kt * (kt - 1) / 2;
+          tek += kt * (static_cast<unsigned long long>(ta) + c.t[Cxz[i]] +
+                       c.t[Cyz[i]] - 6);
+          tdk += kt * (dxy + static_cast<unsigned long long>(dz) - 9);
+        }
+      }
+    }
+    if (cnt >= 2) {
+      const int R = (cnt + 63) >> 6;
+      for (int i = 0; i < cnt; ++i) {
+        const int32_t zi = Cz[i];
+        const int64_t bi = c.drp[zi];
+        const int di = static_cast<int>(c.drp[zi + 1] - bi);
+        const int32_t *Ni = c.dcol + bi;
+        for (int w = 0; w < R; ++w) {
+          const int j = w * 64 + lane;
+          const bool adj = j < cnt && contains_g(Ni, di, Cz[j]);
+          const unsigned long long mask = __ballot(adj);
+          if (lane == 0) rows[int64_t(i) * R + w] = mask;
+        }
+      }
+      wave_sync();  // rows written by lane 0
+      if (R == 1) {
+        for (int i = lane; i < cnt; i += 64) {
+          const unsigned long long ri = rows[i];
+          k4 += static_cast<unsigned long long>(__popcll(ri));
+          for (unsigned long long b = ri; b; b &= b - 1)
+            k5 += static_cast<unsigned long long>(
+                __popcll(ri & rows[__builtin_ctzll(b)]));
+        }
+      } else {
+        for (int i = 0; i < cnt; ++i) {
+          const unsigned long long *ri = rows + int64_t(i) * R;
+          int nj = 0;  // the bits of row i, listed in ML
+          for (int w0 = 0; w0 < R; w0 += 64) {
+            const int w = w0 + lane;
+            const unsigned long long word = w < R ? ri[w] : 0ull;
+            const int pc = __popcll(word);
+            const int incl = wave_incl_scan(pc);
+            int o = nj + incl - pc;
+            for (unsigned long long b = word; b; b &= b - 1)
+              ML[o++] = w * 64 + __builtin_ctzll(b);
+            nj += __shfl(incl, 63, 64);
+          }
+          wave_sync();  // ML written by other lanes
+          if (lane == 0) k4 += static_cast<unsigned long long>(nj);
+          for (int p = lane; p < nj; p += 64) {
+            const unsigned long long *rj = rows + int64_t(ML[p]) * R;
+            unsigned long long s5 = 0;
+            for (int w = 0; w < R; ++w)
+              s5 += static_cast<unsigned long long>(__popcll(ri[w] & rj[w]));
+            k5 += s5;
+          }
+          wave_sync();  // before ML is overwritten
+        }
+      }
+    }
+    wave_sync();  // before the next arc overwrites the slab
+  }
+  kk = wave_sum_u64(kk);
+  tek = wave_sum_u64(tek);
+  tdk = wave_sum_u64(tdk);
+  k4 = wave_sum_u64(k4);
+  k5 = wave_sum_u64(k5);
+  bad = wave_sum_u64(bad);
+  if (lane == 0) {
+    if (kk) atomicAdd(&c.acc[kCensusTriPairs], kk);
+    if (tek) atomicAdd(&c.acc[kCensusTriEdgeK], tek);
+    if (tdk) atomicAdd(&c.acc[kCensusTriDegK], tdk);
+    if (k4) atomicAdd(&c.acc[kCensusK4Rows], k4);
+    if (k5) atomicAdd(&c.acc[kCensusK5], k5);
+    if (bad) atomicAdd(&c.acc[kCensusMismatch], bad);
   }
 }
 
@@ -237,6 +440,18 @@ int census_c4_launch(const CensusArgs &c, hipStream_t s) {
 }
 
 int census_waves(int cus) { return cus * 16; }
+
+int64_t census_k5_slab_len(int64_t max_out, int64_t max_deg) {
+  const int64_t L = std::max<int64_t>(max_out, 1);
+  const int64_t D = std::max<int64_t>(max_deg, 1);
+  return ((4 * L + D + 1) & ~int64_t(1)) + 2 * L * ((L + 63) / 64);
+}
+
+int census_k5_launch(const CensusArgs &c, hipStream_t s) {
+  const int blocks = std::max(c.k5_waves / kWaves, 1);
+  hipLaunchKernelGGL(census_k5, dim3(blocks), dim3(kWaves * 64), 0, s, c);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int census_launch(const CensusArgs &c, int cus, hipStream_t s) {
   const int gv = static_cast<int>(std::max<int64_t>(
