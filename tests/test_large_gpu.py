@@ -41,3 +41,16 @@ def test_p1_p4_rmat18(gpu_available):
     print(f"rmat18 P4 census {p4} kernels {p4k}", flush=True)
     assert p4 == p4k
     dg.close()
+
+
+def test_p7_p8_rmat16_census_matches_kernels(gpu_available):
+    # the census closed forms against the reference-form per-edge kernels
+    # (about 15 s of per-edge kernels) on RMAT-16, ef 16, seed 1
+    dg = ma.DeviceGraph(ma.Graph.rmat(16, 16, seed=1), 0)
+    kern = ma.make_tune(pattern_census=-1)
+    for p in (7, 8):
+        a, st = dg.pattern_count(p)
+        b = dg.pattern_count(p, kern)[0]
+        print(f"P{p} rmat16: census {a} kernels {b}", flush=True)
+        assert st.path == 30 + p and a == b
+    dg.close()
