@@ -177,9 +177,10 @@ __global__ void census_sums(CensusArgs c) {
 // triangles {x, y, z} seen at this arc (each triangle once), kept with the
 // positions of x->z and y->z in the DAG column array.
 //   * k_T = |N(x) ∩ N(y) ∩ N(z)|, the 4-cliques on the triangle, from the
-//     full common neighbourhood F = N(x) ∩ N(y) (|F| must equal t_e);
-//     the triangle adds C(k_T, 2), k_T (t_xy + t_xz + t_yz - 6) and
-//     k_T (d_x + d_y + d_z - 9);
+//     full common neighbourhood F = N(x) ∩ N(y) (|F| must equal t_e; for
+//     small F the membership tests of 64 triangles at a time are spread
+//     over the lanes together); the triangle adds C(k_T, 2),
+//     k_T (t_xy + t_xz + t_yz - 6) and k_T (d_x + d_y + d_z - 9);
 //   * the DAG adjacency inside C as bit rows (bit j of row i: C[j] in
 //     N+(C[i])): row i's bits are the 4-cliques {x, y, C[i], C[j]}, and
 //     |row i ∩ row j| over the bits j of row i are the 5-cliques, each
@@ -190,8 +191,13 @@ __global__ void census_sums(CensusArgs c) {
 constexpr int kSerialKt = 128;  // |F| up to this: one triangle per lane
 
 __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
+  __shared__ int s_off[kWaves][64];
+  __shared__ int s_dz[kWaves][64];
+  __shared__ int64_t s_base[kWaves][64];
+  __shared__ unsigned s_hits[kWaves][64];
   const int lane = lane64();
-  const int64_t wid = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  const int wv = threadIdx.x >> 6;
+  const int64_t wid = int64_t(blockIdx.x) * kWaves + wv;
   const int64_t L = c.slab_len;
   int32_t *Cz = c.k5_slab + wid * c.k5_len;
   uint32_t *Cxz = reinterpret_cast<uint32_t *>(Cz + L);
@@ -263,16 +269,44 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
     wave_sync();  // C and F written by other lanes
     const unsigned long long dxy = static_cast<unsigned long long>(dx + dy);
     if (te <= kSerialKt) {
-      for (int i = lane; i < cnt; i += 64) {
-        const int32_t z = Cz[i];
-        const int dz = c.deg[z];
-        const int32_t *Nz = c.col + c.rowptr[z];
-        unsigned k = 0;
-        if (te <= dz) {
-          for (int q = 0; q < te; ++q) k += contains_g(Nz, dz, F[q]);
-        } else {
-          for (int q = 0; q < dz; ++q) k += contains_g(F, te, Nz[q]);
+      // 64 triangles at a time; their membership tests (the shorter of F
+      // and N(z) searched in the other) flattened over the lanes, the hits
+      // counted per triangle in LDS.
+      for (int i0 = 0; i0 < cnt; i0 += 64) {
+        const int i = i0 + lane;
+        int len = 0, dz = 0;
+        int64_t bz = 0;
+        if (i < cnt) {
+          const int32_t z = Cz[i];
+          dz = c.deg[z];
+          bz = c.rowptr[z];
+          len = te <= dz ? te : dz;
         }
+        const int incl = wave_incl_scan(len);
+        const int total = __shfl(incl, 63, 64);
+        s_off[wv][lane] = incl - len;
+        s_dz[wv][lane] = dz;
+        s_base[wv][lane] = bz;
+        s_hits[wv][lane] = 0;
+        wave_sync();  // the per-triangle LDS entries
+        for (int p = lane; p < total; p += 64) {
+          // r = the last triangle whose run starts at or before p.
+          int lo = 0, hi = 64;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[wv][mid] <= p) lo = mid + 1;
+            else hi = mid;
+          }
+          const int r = lo - 1;
+          const int q = p - s_off[wv][r];
+          const int dr = s_dz[wv][r];
+          const int32_t *Nz = c.col + s_base[wv][r];
+          const bool hit = te <= dr ? contains_g(Nz, dr, F[q])
+                                    : contains_g(F, te, Nz[q]);
+          if (hit) atomicAdd(&s_hits[wv][r], 1u);
+        }
+        wave_sync();  // the hit counts
+        const unsigned k = s_hits[wv][lane];
         if (k) {
           const unsigned long long kt = k;
           kk += kt * (kt - 1) / 2;
@@ -280,6 +314,7 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
                        c.t[Cyz[i]] - 6);
           tdk += kt * (dxy + static_cast<unsigned long long>(dz) - 9);
         }
+        wave_sync();  // before the next 64 triangles overwrite the entries
       }
     } else {
       for (int i = 0; i < cnt; ++i) {
