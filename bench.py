@@ -36,6 +36,7 @@ WORKLOADS = {
     "tc_er22": ("er", 22, 16, 3),
     "tc_rmat20": ("rmat", 20, 16, 4),
     "tc_rmat16": ("rmat", 16, 16, 1),
+    "tc_rmat24": ("rmat", 24, 16, 2),
 }
 
 
