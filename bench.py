@@ -362,7 +362,7 @@ def main() -> int:
     ap.add_argument("--pattern-graph", type=int, nargs=3, default=(20, 16, 4),
                     metavar=("SCALE", "EF", "SEED"),
                     help="patterns mode: the RMAT graph")
-    ap.add_argument("--patterns", default="P1,P4")
+    ap.add_argument("--patterns", default="P1,P4,P7,P8")
     ap.add_argument("--pattern-methods", default="census",
                     help="patterns mode: census and/or kernels")
     ap.add_argument("--no-device-plan", action="store_true",

@@ -54,3 +54,17 @@ def test_p7_p8_rmat16_census_matches_kernels(gpu_available):
         print(f"P{p} rmat16: census {a} kernels {b}", flush=True)
         assert st.path == 30 + p and a == b
     dg.close()
+
+
+def test_p7_p8_rmat18_census(gpu_available):
+    # P7 on RMAT-18, ef 16, seed 4 by the census; the value is the one the
+    # reference-form per-edge kernels return for this graph (77 s of
+    # per-edge kernels, too long for this suite)
+    dg = ma.DeviceGraph(ma.Graph.rmat(18, 16, seed=4), 0)
+    p7, st = dg.pattern_count(7)
+    print(f"rmat18 P7 census {p7} in {st.kernel_ms:.1f} ms", flush=True)
+    assert st.path == 37 and p7 == 17468419210578
+    p8, st = dg.pattern_count(8)
+    print(f"rmat18 P8 census {p8} in {st.kernel_ms:.1f} ms", flush=True)
+    assert st.path == 38 and p8 > 0
+    dg.close()
