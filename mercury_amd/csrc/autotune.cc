@@ -107,7 +107,8 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
     }
   }
   // Leave the winner resident (plan, items, grid cap).
-  if ((rc = tc_replan(dg, tc_params_from(&cur), nullptr))) return rc;
+  if ((rc = tc_replan(dg, tc_params_from(&cur, dg->host->nnz / 2), nullptr)))
+    return rc;
   dg->tc.items = cur.tc_items;
   dg->tc.grid_cap = cur.grid_blocks;
   dg->tc.cursor_min = cur.tc_cursor_min;

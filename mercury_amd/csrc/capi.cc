@@ -35,8 +35,9 @@ size_t TcResident::bytes() const {
   return b;
 }
 
-TcPlanParams tc_params_from(const mgpm_tune *t) {
+TcPlanParams tc_params_from(const mgpm_tune *t, int64_t arcs) {
   TcPlanParams p;
+  if (arcs > 0) p.chunk_work = tc_default_chunk(arcs);
   p.small_max = kTcSmallMaxDeg;
   p.wave_max = kTcWaveMaxDeg;
   p.block_max = kTcBlockMaxDeg;
@@ -386,8 +387,8 @@ int mgpm_dgraph_upload_ex(const mgpm_graph *g, int device, void *stream,
   rc = upload_vec(dg->csr.colidx, G.colidx, s);
   if (rc) return rc;
   rc = (flags & MGPM_UPLOAD_DEVICE_PLAN)
-           ? tc_make_resident_device(dg.get(), tc_params_from(nullptr), s)
-           : tc_make_resident(dg.get(), tc_params_from(nullptr), s);
+           ? tc_make_resident_device(dg.get(), tc_params_from(nullptr, G.nnz / 2), s)
+           : tc_make_resident(dg.get(), tc_params_from(nullptr, G.nnz / 2), s);
   if (rc) return rc;
   *out = dg.release();
   return MGPM_OK;
@@ -425,7 +426,8 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   double prep = 0;
   // NULL keeps the resident plan (the defaults, or what mgpm_autotune
   // chose); an explicit tune rebuilds it when its plan knobs differ.
-  const TcPlanParams want = tune ? tc_params_from(tune) : dg->tc.params;
+  const TcPlanParams want =
+      tune ? tc_params_from(tune, dg->host->nnz / 2) : dg->tc.params;
   if (!same_params(want, dg->tc.params)) {
     const auto tp = std::chrono::steady_clock::now();
     rc = tc_replan(dg, want, s);

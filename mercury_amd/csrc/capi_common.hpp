@@ -134,8 +134,21 @@ struct DagResident {
   }
 };
 
-// Shared by the C-ABI units.
-TcPlanParams tc_params_from(const mgpm_tune *t);
+// Default task-cost chunk for a DAG of `arcs` arcs: 64K below 2^27 arcs,
+// else 64K times the largest power of two <= arcs / 2^26, at most 1M.  On
+// one MI355X, RMAT-22 and ER-22 (just under 2^26 arcs) count fastest with
+// 64K chunks (18.11 vs 18.52 ms at 128K on RMAT-22), RMAT-24 (2.6 x 10^8
+// arcs) with 128K (132 vs 175 ms): the workgroup-table owners of larger
+// graphs rebuild their tables once per chunk.
+inline int64_t tc_default_chunk(int64_t arcs) {
+  int64_t f = 1;
+  while (f < 16 && (arcs >> 26) >= 2 * f) f *= 2;
+  return int64_t(65536) * f;
+}
+
+// Shared by the C-ABI units.  With arcs > 0 the unset chunk size takes
+// tc_default_chunk(arcs).
+TcPlanParams tc_params_from(const mgpm_tune *t, int64_t arcs = 0);
 int set_device(int device);
 int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
                      hipStream_t s);
@@ -148,7 +161,8 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
 inline int default_tc_items(uint64_t probes, uint64_t partners) {
   return probes < 32 * partners ? 4 : 8;
 }
-// P1 and P4 in closed form from a triangle and 4-cycle census (census.cc).
+// P1, P4, P7 and P8 in closed form from a triangle, 4-cycle, 4-clique and
+// 5-clique census (census.cc).
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms);
 // Rebuilds the resident triangle plan where it was built (host or device).
