@@ -81,7 +81,9 @@ typedef struct mgpm_tune {
   int32_t wave_max_deg;
   int32_t block_max_deg;
   int32_t chunk_work;     /* TC: max cost of one task (list items plus 16
-                             per partner); heavier owners are split */
+                             per partner); heavier owners are split
+                             (0 = 65536, doubled per doubling of the DAG's
+                             arcs from 2^27 on, at most 2^20) */
   int32_t grid_blocks;    /* cap on persistent workgroups per kernel */
   int32_t pattern_waves;  /* P1..P8: waves per workgroup */
   int32_t group_work;     /* TC: light tasks are packed into queue entries
