@@ -228,6 +228,8 @@ __device__ __forceinline__ int class_of_slot(int64_t j,
   return c;
 }
 
+// Cost boundaries of the queue groups: a class's first task, and every
+// task at which the class's work prefix crosses a multiple of its limit.
 __global__ void group_flags(const int64_t *__restrict__ wpre, int64_t nt,
                             const TcDplanSummary *__restrict__ sum,
                             int64_t *__restrict__ flag) {
@@ -236,13 +238,37 @@ __global__ void group_flags(const int64_t *__restrict__ wpre, int64_t nt,
     const int64_t off = sum->class_off[c];
     const int64_t r = j - off;
     int64_t f = 1;
-    if (r != 0 && r % 64 != 0) {
+    if (r != 0) {
       const int64_t base = wpre[off], lim = sum->class_limit[c];
       f = (wpre[j] - base) / lim != (wpre[j - 1] - base) / lim ? 1 : 0;
     }
     flag[j] = f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) flag[nt] = 0;
+}
+
+// First task of every cost group (gix: exclusive scan of the cost flags).
+__global__ void group_starts(const int64_t *__restrict__ flag,
+                             const int64_t *__restrict__ gix, int64_t nt,
+                             int64_t *__restrict__ start) {
+  MGPM_GRID_LOOP(j, nt) {
+    if (flag[j]) start[gix[j]] = j;
+  }
+}
+
+// Final boundaries: the cost boundaries, and every 64th task counted from
+// the start of its cost group, so a queue entry holds at most 64 tasks and
+// light tasks fill it (the host builder's greedy cut closes a group at 64
+// tasks the same way).
+__global__ void group_cap_flags(const int64_t *__restrict__ flag,
+                                const int64_t *__restrict__ gix,
+                                const int64_t *__restrict__ start, int64_t nt,
+                                int64_t *__restrict__ out) {
+  MGPM_GRID_LOOP(j, nt) {
+    const int64_t g = gix[j + 1] - 1;  // j's cost group
+    out[j] = (flag[j] || (j - start[g]) % 64 == 0) ? 1 : 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[nt] = 0;
 }
 
 // Group offsets of class c live at groups + class_off[c] + c (its tasks'
@@ -372,8 +398,16 @@ int dplan_tasks(const TcDplanArgs &a, int64_t np, int64_t nt, hipStream_t s) {
                      a.summary, a.gflag);
   if (dplan_scan(a.scan_tmp, a.scan_bytes, a.gflag, a.gix, nt + 1, s))
     return -1;
-  hipLaunchKernelGGL(group_offsets, dim3(g), dim3(kBlock), 0, s, a.gflag,
-                     a.gix, nt, a.summary, a.groups_out);
+  // ca, cb, cas are dead after class_scatter: cost-group starts, final
+  // flags and their scan.
+  hipLaunchKernelGGL(group_starts, dim3(g), dim3(kBlock), 0, s, a.gflag,
+                     a.gix, nt, a.ca);
+  hipLaunchKernelGGL(group_cap_flags, dim3(g), dim3(kBlock), 0, s, a.gflag,
+                     a.gix, a.ca, nt, a.cb);
+  if (dplan_scan(a.scan_tmp, a.scan_bytes, a.cb, a.cas, nt + 1, s))
+    return -1;
+  hipLaunchKernelGGL(group_offsets, dim3(g), dim3(kBlock), 0, s, a.cb,
+                     a.cas, nt, a.summary, a.groups_out);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
