@@ -1,8 +1,11 @@
-// `sm <graph prefix> <P1..P8> [device]` — drop-in for the reference's
-// subgraph-matching CLI (subgraph_matching/main_mpi.cpp:5-38 +
+// `sm <graph prefix> <P1..P8> [device] [--set-algebra]` — drop-in for the
+// reference's subgraph-matching CLI (subgraph_matching/main_mpi.cpp:5-38 +
 // sm_multigpu.cu:97-239) on one GPU, without MPI.  An unknown pattern
 // prints "Not support." and exits with -1 as sm_multigpu.cu:194-198 does.
 // The count is printed under the reference's label "total_num_triangles".
+// P1, P4, P7 and P8 are counted in closed form from a clique/cycle census
+// unless --set-algebra asks for the reference's per-edge set algebra (the
+// same counts).
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -17,8 +20,14 @@ static int die(int rc, const char *what) {
 }
 
 int main(int argc, char **argv) {
+  bool set_algebra = false;
+  if (argc > 3 && std::strcmp(argv[argc - 1], "--set-algebra") == 0) {
+    set_algebra = true;
+    --argc;
+  }
   if (argc < 3) {
-    std::printf("Usage: %s <graph> <pattern> [device]\n", argv[0]);
+    std::printf("Usage: %s <graph> <pattern> [device] [--set-algebra]\n",
+                argv[0]);
     std::printf("Example: %s /graph_inputs/mico/graph P1\n", argv[0]);
     return 1;
   }
@@ -40,7 +49,11 @@ int main(int argc, char **argv) {
   if (rc) return die(rc, "upload");
   uint64_t total = 0;
   mgpm_stats st;
-  rc = mgpm_pattern_count(dg, pattern, nullptr, nullptr, &total, &st);
+  mgpm_tune tune;
+  std::memset(&tune, 0, sizeof(tune));
+  tune.pattern_census = -1;
+  rc = mgpm_pattern_count(dg, pattern, set_algebra ? &tune : nullptr, nullptr,
+                          &total, &st);
   if (rc == MGPM_ERR_PATTERN) {
     std::printf("Not support.\n");
     std::exit(-1);

@@ -86,11 +86,15 @@ def test_unknown_pattern_rejected(gpu_available):
     assert ei.value.code == -8
 
 
-def test_sm_cli_matches_golden(gpu_available):
+@pytest.mark.parametrize("algebra", [[], ["--set-algebra"]])
+def test_sm_cli_matches_golden(gpu_available, algebra):
+    """Default (P1, P4, P7, P8 by census, the rest per edge) and
+    --set-algebra (every pattern per edge)."""
     want = _golden()["gnp40"]["counts"]
-    for p in range(1, 9):  # P1, P4, P7, P8 by census, the rest per edge
+    for p in range(1, 9):
         r = subprocess.run([str(ROOT / "mercury_amd" / "bin" / "sm"),
-                            str(GOLDEN / "gnp40" / "graph"), f"P{p}"],
+                            str(GOLDEN / "gnp40" / "graph"), f"P{p}"]
+                           + algebra,
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 0, r.stderr
         line = [l for l in r.stdout.splitlines()
