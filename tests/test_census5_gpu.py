@@ -118,3 +118,24 @@ def test_census5_dense_mix(gpu_available):
         print(f"P{p} dense mix: census {a} kernels {b} oracle {want}",
               flush=True)
         assert a == want and b == want, p
+
+
+def test_census5_dense_random_golden(gpu_available):
+    # dense G(n, p) graphs whose oracle counts are committed
+    # (tests/golden/make_dense_counts.py)
+    import json
+    import sys
+    from pathlib import Path
+    golden = Path(__file__).resolve().parent / "golden"
+    sys.path.insert(0, str(golden))
+    from make_dense_counts import dense_edges
+    for row in json.loads((golden / "dense_counts.json").read_text()):
+        n = row["n"]
+        rp, ci = csr_from_edges(n, dense_edges(n, row["p"], row["seed"]))
+        assert len(ci) // 2 == row["edges"]
+        g = ma.Graph.from_csr(rp, ci)
+        for p in (7, 8):
+            a, b = _both(g, p)
+            print(f"P{p} G({n}, {row['p']}): census {a} kernels {b} "
+                  f"oracle {row[f'P{p}']}", flush=True)
+            assert a == b == row[f"P{p}"], (n, p)
