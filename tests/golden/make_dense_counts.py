@@ -1,8 +1,9 @@
-"""Writes tests/golden/dense_counts.json: P7 and P8 of dense random graphs,
-computed by the CPU oracle (oracle/oracle.c), for the GPU census tests
+"""Writes tests/golden/dense_counts.json: P1, P4, P7 and P8 of dense random
+graphs, computed by the CPU oracle (oracle/oracle.c), for the GPU census tests
 (tests/test_census5_gpu.py).  An edge of G(n, p) lies on about n p^2
 triangles (49 to 128 here), so the census's triangle sets cross 64 (its
-multi-word adjacency rows) and, at p = 0.8, about half the common
+multi-word adjacency rows; for P1 and P4 its K4 step intersects per
+triangle once a set exceeds 16) and, at p = 0.8, about half the common
 neighbourhoods exceed 128 (its wave-per-triangle path), on random rather
 than clique-shaped neighbourhoods.
 
@@ -40,7 +41,7 @@ def main() -> None:
     for n, p, seed in CASES:
         rp, ci = csr_from_edges(n, dense_edges(n, p, seed))
         row = {"n": n, "p": p, "seed": seed, "edges": int(len(ci) // 2)}
-        for pat in (7, 8):
+        for pat in (1, 4, 7, 8):
             row[f"P{pat}"] = oracle.pattern(rp, ci, pat)[0]
         print(row, flush=True)
         out.append(row)

@@ -120,9 +120,9 @@ def test_census5_dense_mix(gpu_available):
         assert a == want and b == want, p
 
 
-def test_census5_dense_random_golden(gpu_available):
+def test_census_dense_random_golden(gpu_available):
     # dense G(n, p) graphs whose oracle counts are committed
-    # (tests/golden/make_dense_counts.py)
+    # (tests/golden/make_dense_counts.py); P1 and P4 by the census too
     import json
     import sys
     from pathlib import Path
@@ -134,7 +134,7 @@ def test_census5_dense_random_golden(gpu_available):
         rp, ci = csr_from_edges(n, dense_edges(n, row["p"], row["seed"]))
         assert len(ci) // 2 == row["edges"]
         g = ma.Graph.from_csr(rp, ci)
-        for p in (7, 8):
+        for p in (1, 4, 7, 8):
             a, b = _both(g, p)
             print(f"P{p} G({n}, {row['p']}): census {a} kernels {b} "
                   f"oracle {row[f'P{p}']}", flush=True)
