@@ -27,7 +27,7 @@ namespace mgpm {
 namespace {
 
 constexpr int kWaves = 4;
-constexpr int kPairMax = 16;  // |C| up to this: K4 test by flattened pairs
+constexpr int kPairMax = 64;  // |C| up to this: K4 test by flattened pairs
 
 // First index in sorted list[0, len) with list[i] >= key.
 __device__ __forceinline__ int lower_bound_g(const int32_t *__restrict__ list,
