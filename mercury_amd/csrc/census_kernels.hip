@@ -339,19 +339,39 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
     }
     if (cnt >= 2) {
       const int R = (cnt + 63) >> 6;
-      for (int i = 0; i < cnt; ++i) {
-        const int32_t zi = Cz[i];
-        const int64_t bi = c.drp[zi];
-        const int di = static_cast<int>(c.drp[zi + 1] - bi);
-        const int32_t *Ni = c.dcol + bi;
-        for (int w = 0; w < R; ++w) {
-          const int j = w * 64 + lane;
-          const bool adj = j < cnt && contains_g(Ni, di, Cz[j]);
+      if (cnt <= 32) {
+        // 64 / cnt rows per ballot: lane l tests C[l % cnt] against row
+        // i0 + l / cnt, and row i0 + r is bits [r cnt, (r + 1) cnt).
+        const int rpb = 64 / cnt;
+        const int li = lane / cnt, j = lane - li * cnt;
+        for (int i0 = 0; i0 < cnt; i0 += rpb) {
+          const int i = i0 + li;
+          bool adj = false;
+          if (li < rpb && i < cnt) {
+            const int32_t zi = Cz[i];
+            const int64_t bi = c.drp[zi];
+            const int di = static_cast<int>(c.drp[zi + 1] - bi);
+            adj = contains_g(c.dcol + bi, di, Cz[j]);
+          }
           const unsigned long long mask = __ballot(adj);
-          if (lane == 0) rows[int64_t(i) * R + w] = mask;
+          if (lane < rpb && i0 + lane < cnt)
+            rows[i0 + lane] = (mask >> (lane * cnt)) & ((1ull << cnt) - 1);
+        }
+      } else {
+        for (int i = 0; i < cnt; ++i) {
+          const int32_t zi = Cz[i];
+          const int64_t bi = c.drp[zi];
+          const int di = static_cast<int>(c.drp[zi + 1] - bi);
+          const int32_t *Ni = c.dcol + bi;
+          for (int w = 0; w < R; ++w) {
+            const int j = w * 64 + lane;
+            const bool adj = j < cnt && contains_g(Ni, di, Cz[j]);
+            const unsigned long long mask = __ballot(adj);
+            if (lane == 0) rows[int64_t(i) * R + w] = mask;
+          }
         }
       }
-      wave_sync();  // rows written by lane 0
+      wave_sync();  // rows written by other lanes
       if (R == 1) {
         for (int i = lane; i < cnt; i += 64) {
           const unsigned long long ri = rows[i];
