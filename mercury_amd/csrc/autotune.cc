@@ -126,6 +126,9 @@ int tune_pattern(mgpm_dgraph *dg, int pattern, int reps, mgpm_tune *best,
   if (rc) return rc;
   cur.grid_blocks = dg->pat.grid_cap;
   cur.pattern_split = dg->pat.plan[pattern].split;
+  // The knobs tuned here steer the per-edge kernels (shares and grid); the
+  // census ignores them, so every candidate is timed on those kernels.
+  cur.pattern_census = -1;
   uint64_t want = 0;
   double cur_ms = 0;
   if ((rc = time_pattern(dg, pattern, cur, reps, &want, &cur_ms))) return rc;
@@ -180,7 +183,7 @@ int autotune_size_classes(const std::vector<mgpm_dgraph *> &graphs, int reps) {
     double ms = 0;
     int rc = tune_tc(todo[i], reps, &t, &ms);  // the class's largest graph
     if (rc) return rc;
-    todo[i]->tc.generation++;
+    todo[i]->tc.generation = next_generation();
     const TcPlanParams p = tc_params_from(&t);
     for (size_t k = i + 1; k < j; ++k) {
       mgpm_dgraph *g = todo[k];
@@ -211,7 +214,7 @@ extern "C" int mgpm_autotune(mgpm_dgraph *dg, int pattern, int32_t reps,
     rc = pattern == MGPM_TC ? tune_tc(dg, reps, &t, &ms)
                             : tune_pattern(dg, pattern, reps, &t, &ms);
     if (rc) return rc;
-    dg->tc.generation++;  // batches keyed on this re-plan
+    dg->tc.generation = next_generation();  // batches keyed on this re-plan
     if (best) *best = t;
     if (best_ms) *best_ms = ms;
     return MGPM_OK;

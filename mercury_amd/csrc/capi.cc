@@ -2,6 +2,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 
@@ -14,6 +15,11 @@ thread_local std::string g_last_error;
 }
 
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+uint64_t next_generation() {
+  static std::atomic<uint64_t> seq{0};
+  return seq.fetch_add(1, std::memory_order_relaxed) + 1;
+}
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -116,7 +122,7 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   r.dev = p;
   r.params = params;
   r.on_device = false;
-  r.generation++;
+  r.generation = next_generation();
   r.s_plus = plan.s_plus;
   r.probes = plan.probes;
   r.m_plus = static_cast<uint64_t>(plan.m_plus);
@@ -405,7 +411,11 @@ int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
 int mgpm_dgraph_free(mgpm_dgraph *dg) {
   if (!dg) return MGPM_OK;
   (void)hipSetDevice(dg->device);
-  (void)hipDeviceSynchronize();
+  // Every call on the handle synchronises its own stream before returning,
+  // so no work of it is in flight here; batches and servers that hold the
+  // graph must be done with it (their own free/destroy waits for them).
+  // No device-wide synchronisation: a server freeing one graph would stall
+  // every batch in flight on the device.
   if (dg->ev0) (void)hipEventDestroy(dg->ev0);
   if (dg->ev1) (void)hipEventDestroy(dg->ev1);
   if (dg->pinned) (void)hipHostFree(dg->pinned);

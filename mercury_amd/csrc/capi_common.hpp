@@ -71,8 +71,9 @@ struct DevBuf {
 // Triangle-count state resident on one device.
 struct TcResident {
   TcPlanParams params;
-  uint64_t generation = 0;  // bumped whenever any plan of the graph (TC or
-                            // pattern) or its launch settings change
+  uint64_t generation = 0;  // renewed (next_generation) whenever any plan
+                            // of the graph (TC or pattern) or its launch
+                            // settings change
   int items = 8;            // list items per lane per step
   int grid_cap = 0;         // cap on persistent workgroups (0 = none)
   int cursor_min = -1;      // owner-cursor streaming threshold (<= 0: off)
@@ -145,6 +146,12 @@ inline int64_t tc_default_chunk(int64_t arcs) {
   while (f < 16 && (arcs >> 26) >= 2 * f) f *= 2;
   return int64_t(65536) * f;
 }
+
+// A value no plan generation of any graph in this process has taken yet.
+// Batches key their recorded plans on (graph, pattern, generation); a
+// process-wide sequence keeps a graph uploaded at the address of a freed
+// one from matching the freed graph's key.
+uint64_t next_generation();
 
 // Shared by the C-ABI units.  With arcs > 0 the unset chunk size takes
 // tc_default_chunk(arcs).

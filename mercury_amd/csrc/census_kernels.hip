@@ -20,6 +20,7 @@
 #include <algorithm>
 
 #include "census.hpp"
+#include "choose.hpp"
 #include "wave.hpp"
 
 namespace mgpm {
@@ -150,13 +151,13 @@ __global__ void census_sums(CensusArgs c) {
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < c.n;
        v += int64_t(gridDim.x) * blockDim.x) {
     const unsigned long long d = static_cast<unsigned long long>(c.deg[v]);
-    s3 += d >= 3 ? d * (d - 1) * (d - 2) / 6 : 0;
+    s3 += choose3(d);
     for (int64_t a = c.drp[v]; a < c.drp[v + 1]; ++a) {
       const unsigned long long t = c.t[a];
       const unsigned long long dy =
           static_cast<unsigned long long>(c.deg[c.dcol[a]]);
-      s2 += t * (t - 1) / 2;
-      st += t >= 3 ? t * (t - 1) * (t - 2) / 6 : 0;
+      s2 += choose2(t);
+      st += choose3(t);
       sp += (d - t - 1) * (dy - t - 1);
     }
   }

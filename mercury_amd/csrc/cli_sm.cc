@@ -1,11 +1,14 @@
-// `sm <graph prefix> <P1..P8> [device] [--set-algebra]` — drop-in for the
-// reference's subgraph-matching CLI (subgraph_matching/main_mpi.cpp:5-38 +
-// sm_multigpu.cu:97-239) on one GPU, without MPI.  An unknown pattern
-// prints "Not support." and exits with -1 as sm_multigpu.cu:194-198 does.
-// The count is printed under the reference's label "total_num_triangles".
-// P1, P4, P7 and P8 are counted in closed form from a clique/cycle census
-// unless --set-algebra asks for the reference's per-edge set algebra (the
-// same counts).
+// `sm <graph prefix> <P1..P8> [devices per node] [--device N]
+// [--set-algebra]` -- drop-in for the reference's subgraph-matching CLI
+// (subgraph_matching/main_mpi.cpp:5-38 + sm_multigpu.cu:97-239) on one
+// GPU, without MPI.  The third argument means what it means there:
+// devices per node, 0 for all, and the one process (rank 0) takes device
+// 0 % count (sm_multigpu.cu:100-104); `--device N` picks another device.
+// An unknown pattern prints "Not support." and exits with -1 as
+// sm_multigpu.cu:194-198 does.  The count is printed under the reference's
+// label "total_num_triangles".  Patterns with a closed form are counted
+// from a subgraph census unless --set-algebra asks for the reference's
+// per-edge set algebra (the same counts).
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -21,28 +24,45 @@ static int die(int rc, const char *what) {
 
 int main(int argc, char **argv) {
   bool set_algebra = false;
-  if (argc > 3 && std::strcmp(argv[argc - 1], "--set-algebra") == 0) {
-    set_algebra = true;
-    --argc;
+  int device = -1;
+  // Options may follow the positional arguments in any order.
+  int npos = 0;
+  char *pos[4] = {nullptr, nullptr, nullptr, nullptr};
+  for (int i = 0; i < argc; ++i) {
+    if (std::strcmp(argv[i], "--set-algebra") == 0) {
+      set_algebra = true;
+    } else if (std::strcmp(argv[i], "--device") == 0 && i + 1 < argc) {
+      device = std::atoi(argv[++i]);
+    } else if (npos < 4) {
+      pos[npos++] = argv[i];
+    }
   }
-  if (argc < 3) {
-    std::printf("Usage: %s <graph> <pattern> [device] [--set-algebra]\n",
-                argv[0]);
-    std::printf("Example: %s /graph_inputs/mico/graph P1\n", argv[0]);
+  if (npos < 3) {
+    std::printf("Usage: %s <graph> <pattern> [devices per node] "
+                "[--device N] [--set-algebra]\n", argv[0]);
+    std::printf("Example: %s /graph_inputs/mico/graph P1 4\n", argv[0]);
     return 1;
   }
   std::printf("Subgraph matching \n");
   int pattern = -1;
-  if (std::strlen(argv[2]) == 2 && argv[2][0] == 'P' && argv[2][1] >= '1' &&
-      argv[2][1] <= '8')
-    pattern = argv[2][1] - '0';
-  const int device = argc >= 4 ? std::atoi(argv[3]) : 0;
+  if (std::strlen(pos[2]) == 2 && pos[2][0] == 'P' && pos[2][1] >= '1' &&
+      pos[2][1] <= '8')
+    pattern = pos[2][1] - '0';
   mgpm_graph *g = nullptr;
-  int rc = mgpm_graph_load(argv[1], &g);
+  int rc = mgpm_graph_load(pos[1], &g);
   if (rc) return die(rc, "load");
   if (pattern < 0) {
     std::printf("Not support.\n");
     std::exit(-1);
+  }
+  if (device < 0) {
+    int per_node = npos >= 4 ? std::atoi(pos[3]) : 0;
+    if (per_node <= 0) {
+      rc = mgpm_device_count(&per_node);
+      if (rc) return die(rc, "device count");
+      if (per_node <= 0) return die(MGPM_ERR_NODEVICE, "device count");
+    }
+    device = 0 % per_node;  // rank 0 of one process
   }
   mgpm_dgraph *dg = nullptr;
   rc = mgpm_dgraph_upload(g, device, nullptr, &dg);

@@ -55,7 +55,7 @@ int pattern_make_resident(mgpm_dgraph *dg, hipStream_t s, int pattern,
   pr.split = split;
   pr.ready = true;
   r.cap = std::max<int64_t>(plan.max_degree, 64);
-  if (replace) dg->tc.generation++;  // batches holding the old plan re-plan
+  if (replace) dg->tc.generation = next_generation();  // batches holding the old plan re-plan
   return MGPM_OK;
 }
 

@@ -55,6 +55,8 @@ struct mgpm_server {
   mgpm_server_stats st{};
   Clock::time_point busy_since;
   int inflight = 0;
+  int waiters = 0;       // threads inside mgpm_server_wait
+  bool closing = false;  // mgpm_server_destroy has failed what was left
 };
 
 namespace {
@@ -101,13 +103,20 @@ void scheduler(mgpm_server *s) {
   std::unique_lock<std::mutex> lk(s->mu);
   while (!s->stop.load()) {
     bool progressed = false;
-    // Completions.
+    // Completions.  A slot whose work ended in an error (a failed launch
+    // or kernel reported by the event) finishes its requests with
+    // MGPM_ERR_HIP instead of staying busy forever.
     for (Slot &sl : s->slots) {
-      if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
       const auto now = Clock::now();
       const int nb = static_cast<int>(sl.tickets.size());
+      if (q != hipSuccess) set_error(std::string("server batch: ") +
+                                     hipGetErrorString(q));
       for (int i = 0; i < nb; ++i)
-        finish(s, sl.tickets[i], sl.exec.count(i), MGPM_OK, nb, now);
+        finish(s, sl.tickets[i], q == hipSuccess ? sl.exec.count(i) : 0,
+               q == hipSuccess ? MGPM_OK : MGPM_ERR_HIP, nb, now);
       sl.tickets.clear();
       sl.busy = false;
       if (--s->inflight == 0)
@@ -243,10 +252,13 @@ int mgpm_server_wait(mgpm_server *s, uint64_t ticket, int32_t timeout_ms,
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end()) return fail(MGPM_ERR_NOTFOUND, "unknown ticket");
   auto ready = [&] { return s->reqs[ticket].done; };
+  ++s->waiters;
+  bool ok = true;
   if (timeout_ms < 0) s->cv_done.wait(lk, ready);
-  else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms),
-                                ready))
-    return fail(MGPM_ERR_TIMEOUT, "request still running");
+  else ok = s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                                ready);
+  if (--s->waiters == 0 && s->closing) s->cv_done.notify_all();
+  if (!ok) return fail(MGPM_ERR_TIMEOUT, "request still running");
   const Req r = s->reqs[ticket];
   s->reqs.erase(ticket);
   if (out) {
@@ -315,6 +327,18 @@ int mgpm_server_destroy(mgpm_server *s) {
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
     if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  {
+    // Requests still pending or in flight fail; wake their waiters, and
+    // wait until every waiter has left before the server goes away.
+    std::unique_lock<std::mutex> lk(s->mu);
+    const auto now = Clock::now();
+    for (auto &kv : s->reqs)
+      if (!kv.second.done) finish(s, kv.first, 0, MGPM_ERR_FAILED, 0, now);
+    s->pending.clear();
+    s->closing = true;
+    s->cv_done.notify_all();
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
   }
   s->slots.clear();  // lane streams and workspaces
   for (mgpm_dgraph *dg : s->graphs) mgpm_dgraph_free(dg);

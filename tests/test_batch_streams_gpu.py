@@ -88,3 +88,23 @@ def test_batch_empty_and_bad_pattern(gpu_available):
     assert ei.value.code == -8
     # still usable afterwards
     assert b.run([(dg, 0)])[0] == [dg.tc_count()[0]]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_after_graph_freed_and_replaced(gpu_available, use_graph):
+    """A graph freed and replaced by another (glibc often hands the new
+    handle the freed one's address) must not match the batch's recorded
+    plan: the rerun counts the new graph."""
+    g1 = ma.Graph.rmat(9, 8, seed=71)
+    g2 = ma.Graph.rmat(10, 16, seed=72)
+    b = ma.Batch(0, lanes=2, use_graph=use_graph)
+    for _ in range(3):
+        dg = ma.DeviceGraph(g1, 0)
+        c1, _ = b.run([(dg, 0), (dg, 4)])
+        assert c1 == [dg.tc_count()[0], dg.pattern_count(4)[0]]
+        dg.close()
+        dg = ma.DeviceGraph(g2, 0)
+        c2, _ = b.run([(dg, 0), (dg, 4)])
+        rp, ci = g2.csr()
+        assert c2 == [oracle.tc(rp, ci), oracle.pattern(rp, ci, 4)[0]]
+        dg.close()

@@ -1,6 +1,7 @@
 """Device-built degree-oriented DAG and small/large division
-(mgpm_dgraph_orientation) against the host build (itself bit-equal to the
-reference's graph.cc, see test_host.py), array for array."""
+(mgpm_dgraph_orientation) against the oracle's restatement of graph.cc:196-321
+(oracle.orientation, itself bit-equal to the reference's own graph.cc where
+that builds) and against the host build, array for array."""
 import json
 from pathlib import Path
 
@@ -8,6 +9,7 @@ import numpy as np
 import pytest
 
 import mercury_amd as ma
+import oracle
 from conftest import csr_from_edges
 
 pytestmark = pytest.mark.gpu
@@ -18,10 +20,13 @@ KEYS = ("dag_rowptr", "dag_colidx", "small_rowptr", "small_colidx",
 
 
 def _same(g, thd):
-    want = g.orientation(thd)
+    rp, ci = g.csr()
+    want = oracle.orientation(rp, ci, thd)
+    host = g.orientation(thd)
     got, st = ma.DeviceGraph(g, 0).orientation(thd)
     for k in KEYS:
         assert np.array_equal(got[k], want[k]), (k, thd)
+        assert np.array_equal(host[k], want[k]), (k, thd)
     assert st.m_plus == len(want["dag_colidx"]) and st.kernel_ms > 0
 
 
@@ -52,9 +57,10 @@ def test_dag_edge_cases(gpu_available):
 def test_dag_repeat_and_threshold_change(gpu_available):
     g = ma.Graph.rmat(11, 16, seed=5)
     dg = ma.DeviceGraph(g, 0)
+    rp, ci = g.csr()
     for thd in (128, 16, 128):
         got, _ = dg.orientation(thd)
-        want = g.orientation(thd)
+        want = oracle.orientation(rp, ci, thd)
         for k in KEYS:
             assert np.array_equal(got[k], want[k]), (k, thd)
     # the triangle count still works on the same handle afterwards

@@ -92,8 +92,11 @@ def test_sm_cli_matches_golden(gpu_available, algebra):
     --set-algebra (every pattern per edge)."""
     want = _golden()["gnp40"]["counts"]
     for p in range(1, 9):
+        # the reference's form: <graph> <pattern> <devices per node>
+        # (main_mpi.cpp:20-25); 4 devices per node is device 0 on rank 0
         r = subprocess.run([str(ROOT / "mercury_amd" / "bin" / "sm"),
                             str(GOLDEN / "gnp40" / "graph"), f"P{p}"]
+                           + (["4"] if p % 2 else ["--device", "0"])
                            + algebra,
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 0, r.stderr
