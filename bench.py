@@ -238,8 +238,9 @@ def bench_dag(ma, args) -> int:
 def bench_patterns(ma, args) -> int:
     """P1..P8 counts on one RMAT graph (default RMAT-20, ef 16, seed 4, the
     pattern configuration): device time per count for each requested
-    pattern and method ("census": P1, P4, P7, P8 closed forms; "kernels": the
-    reference's per-edge set algebra), and whether the methods agree."""
+    pattern and method ("census": closed forms from the subgraph census;
+    "kernels": the reference's per-edge set algebra), and whether the
+    methods agree."""
     scale, ef, seed = args.pattern_graph
     names = [p.strip() for p in args.patterns.split(",")]
     methods = [m.strip() for m in args.pattern_methods.split(",")]
@@ -252,8 +253,6 @@ def bench_patterns(ma, args) -> int:
         p = ma.PATTERN_NAMES[name]
         counts = {}
         for meth in methods:
-            if meth == "census" and p not in (1, 4, 7, 8):
-                continue
             tune = ma.make_tune(pattern_census=-1) if meth == "kernels" else None
             dg.pattern_count(p, tune)  # builds plans outside the timing
             ks = []
@@ -266,7 +265,7 @@ def bench_patterns(ma, args) -> int:
             rows[f"{name}_{meth}_ms"] = sorted(ks)[len(ks) // 2]
             rows[f"{name}_count"] = c
         agree = agree and len(set(counts.values())) <= 1
-    first = f"{names[0]}_{methods[0] if names[0] in ('P1', 'P4', 'P7', 'P8') else 'kernels'}_ms"
+    first = f"{names[0]}_{methods[0]}_ms"
     out = {
         "metric": "pattern count time", "value": rows.get(first),
         "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1,

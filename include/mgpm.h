@@ -98,10 +98,11 @@ typedef struct mgpm_tune {
                              estimated pair tests exceed 2^pattern_split is
                              cut into shares run by different waves
                              (default 20; -1 = never) */
-  int32_t pattern_census; /* P1, P4, P7, P8: 0 (default) or 1 = closed
-                             form from a triangle, 4-cycle, 4-clique and
-                             5-clique census; -1 = the reference's per-edge
-                             set algebra */
+  int32_t pattern_census; /* P1..P8: 0 (default) or 1 = closed form from
+                             a subgraph census (triangles, 4-cycles, 4- and
+                             5-cliques; for P2, P3, P5, P6 also wheels,
+                             K2,3 and K2,3 plus an edge); -1 = the
+                             reference's per-edge set algebra (same count) */
 } mgpm_tune;
 
 /* Per-call measurements. */
@@ -271,7 +272,10 @@ typedef struct mgpm_batch_opts {
                          triangle knobs of the largest untuned graph of each
                          size class (log2 of DAG arcs) with mgpm_autotune's
                          search and apply them to the rest of the class */
-  int32_t reserved[5];
+  int32_t pattern_census; /* P1..P8: 0 = closed form from the census (each
+                             lane keeps a census workspace), -1 = the
+                             per-edge kernels; the same counts */
+  int32_t reserved[4];
 } mgpm_batch_opts;
 
 /* opts may be NULL (defaults). */
@@ -295,16 +299,22 @@ int mgpm_batch_free(mgpm_batch *b);
  * mgpm_batch_run machinery over `lanes` streams), so requests that arrive
  * while others run join the next batch and batches on different slots
  * overlap on the device.  Admission is by HBM footprint: a graph whose
- * resident footprint would exceed the budget is refused (MGPM_ERR_CAPACITY)
- * and slot workspaces are charged against the same budget.  slots = 1,
- * lanes = 1, max_batch = 1 gives one launch per request in arrival order. */
+ * resident footprint would exceed the budget is refused (MGPM_ERR_CAPACITY),
+ * and slot workspaces are charged against the same budget: a free slot
+ * takes pending requests in arrival order while their workspaces fit, and
+ * the first that does not fit waits for memory to free (batches in flight
+ * to finish, idle workspaces to be released); only a request that cannot
+ * fit even then fails, alone (MGPM_ERR_CAPACITY).  slots = 1, lanes = 1,
+ * max_batch = 1 gives one launch per request in arrival order. */
 typedef struct mgpm_server_config {
   int32_t device;
   int32_t slots;       /* batches in flight (0 = 3) */
   int32_t max_batch;   /* requests per batch (0 = 1024) */
   int32_t lanes;       /* streams per slot (0 = 2) */
   uint64_t hbm_budget; /* bytes for graphs + workspaces (0 = 90% of free) */
-  uint64_t reserved[2];
+  int32_t pattern_census; /* as in mgpm_batch_opts */
+  int32_t reserved32;
+  uint64_t reserved[1];
 } mgpm_server_config;
 
 typedef struct mgpm_request {
@@ -321,6 +331,10 @@ typedef struct mgpm_server_stats {
   uint64_t submitted, completed, failed, launches;
   uint64_t resident_bytes, workspace_bytes, budget_bytes;
   double busy_ms; /* wall time with at least one launch in flight */
+  uint64_t deferred;        /* times a request waited for memory to free
+                               (the intake stopped at it) */
+  uint64_t deferred_failed; /* requests that could not fit even with every
+                               idle workspace released (MGPM_ERR_CAPACITY) */
 } mgpm_server_stats;
 
 int mgpm_server_create(const mgpm_server_config *cfg, mgpm_server **out);

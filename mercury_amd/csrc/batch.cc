@@ -2,9 +2,11 @@
 // which runs one graph and one pattern per process), and the multi-query
 // launcher it shares with the server.
 #include "batch_exec.hpp"
+#include "census.hpp"
 
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <numeric>
 
 using namespace mgpm;
@@ -17,6 +19,12 @@ namespace {
 // device but kept under this many bytes.
 constexpr uint64_t kLaneSlabBudget = uint64_t(2) << 30;
 constexpr int kPatternLanes = 4;
+// Pair-pass arrays per lane (the census workspace's budget).
+constexpr size_t kLaneCensusBudget = size_t(4) << 30;
+// Per query in the scratch buffer: 2 u64 counters and 4 u32 queue heads
+// (zeroed by the fill kernel), and separately the census accumulators.
+constexpr size_t kQueryHead = 32;
+constexpr size_t kQueryAcc = kCensusSlots * sizeof(unsigned long long);
 
 uint64_t slab_bytes(int64_t cap, int grid) {
   return uint64_t(pattern_waves_per_block()) * 8 * uint64_t(cap) *
@@ -35,6 +43,23 @@ uint64_t query_cost(const mgpm_dgraph *g, int pattern) {
   return c + 1;
 }
 
+namespace {
+
+// Census work relative to the triangle count's: the passes each pattern
+// runs over the same DAG (triangles and sums; 4-cycles; 4- and 5-cliques;
+// the pair passes).
+uint64_t census_cost(const mgpm_dgraph *g, int pattern) {
+  const uint64_t tri = g->tc.probes + g->tc.m_plus + 1;
+  switch (pattern) {
+    case MGPM_P1: return 2 * tri;
+    case MGPM_P4: return 3 * tri;
+    case MGPM_P7: case MGPM_P8: return 8 * tri;
+    default: return 24 * tri;
+  }
+}
+
+}  // namespace
+
 BatchExec::~BatchExec() {
   if (lanes_.empty() && !fork_ && !pinned_) return;
   (void)hipSetDevice(device_);
@@ -45,9 +70,10 @@ BatchExec::~BatchExec() {
   if (pinned_) (void)hipHostFree(pinned_);
 }
 
-int BatchExec::init(int device, int cus, int lanes) {
+int BatchExec::init(int device, int cus, int lanes, bool census) {
   device_ = device;
   cus_ = cus;
+  census_ = census;
   lanes = std::max(1, std::min(lanes, 16));
   MGPM_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
   for (int l = 0; l < lanes; ++l) {
@@ -59,6 +85,8 @@ int BatchExec::init(int device, int cus, int lanes) {
     lane_done_.push_back(e);
   }
   slab_.reset(new DevBuf[static_cast<size_t>(lanes)]);
+  census_work_.reset(new CensusWork[static_cast<size_t>(lanes)]);
+  for (int l = 0; l < lanes; ++l) census_work_[l].budget = kLaneCensusBudget;
   slab_cap_.assign(static_cast<size_t>(lanes), 0);
   slab_grid_.assign(static_cast<size_t>(lanes), 0);
   order_.assign(static_cast<size_t>(lanes), {});
@@ -67,33 +95,101 @@ int BatchExec::init(int device, int cus, int lanes) {
 
 uint64_t BatchExec::footprint() const {
   uint64_t b = scratch_.bytes;
-  for (size_t l = 0; l < lanes_.size(); ++l) b += slab_[l].bytes;
+  for (size_t l = 0; l < lanes_.size(); ++l)
+    b += slab_[l].bytes + census_work_[l].bytes();
   return b;
+}
+
+void BatchExec::query_need(const mgpm_query &q, uint64_t *want_census,
+                           uint64_t *want_slab) const {
+  const mgpm_dgraph *g = q.graph;
+  const int p = q.pattern;
+  if (!g || p <= MGPM_TC || p > MGPM_P8) return;
+  if (census_) {
+    *want_census = std::max(*want_census,
+                            census_work_bytes(g, p, kLaneCensusBudget));
+  } else {
+    const int64_t cap = std::max<int64_t>(g->host->max_degree(), 64);
+    const uint64_t per_block = slab_bytes(cap, 1);
+    const uint64_t grid = std::max<uint64_t>(
+        1, std::min<uint64_t>(uint64_t(cus_) * 4, kLaneSlabBudget / per_block));
+    *want_slab = std::max(*want_slab, per_block * grid);
+  }
+}
+
+uint64_t BatchExec::growth(const mgpm_query *q, int nq) const {
+  uint64_t want_census = 0, want_slab = 0;
+  for (int i = 0; i < nq; ++i) query_need(q[i], &want_census, &want_slab);
+  return growth_for(want_census, want_slab, nq);
+}
+
+void BatchExec::release() {
+  (void)hipSetDevice(device_);
+  for (hipStream_t s : lanes_) (void)hipStreamSynchronize(s);
+  for (size_t l = 0; l < lanes_.size(); ++l) {
+    slab_[l].reset();
+    slab_cap_[l] = 0;
+    slab_grid_[l] = 0;
+    census_work_[l].~CensusWork();
+    new (&census_work_[l]) CensusWork();
+    census_work_[l].budget = kLaneCensusBudget;
+  }
+}
+
+uint64_t BatchExec::growth_for(uint64_t want_census, uint64_t want_slab,
+                               int nq) const {
+  // A bound: every lane grown to the largest query's workspace.
+  uint64_t g = 0;
+  const int use = std::min(lanes(), std::max(nq, 1));
+  for (int l = 0; l < use; ++l) {
+    const uint64_t hc = census_work_[l].bytes(), hs = slab_[l].bytes;
+    if (want_census > hc) g += want_census - hc;
+    if (want_slab > hs) g += want_slab - hs;
+  }
+  const uint64_t sc = (kQueryHead + kQueryAcc) * uint64_t(std::max(nq, 64));
+  if (sc > scratch_.bytes) g += sc - scratch_.bytes;
+  return g;
 }
 
 int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
   const int L = lanes();
+  for (int i = 0; i < nq; ++i) {
+    if (!q[i].graph) return fail(MGPM_ERR_INVALID, "query without a graph");
+    if (q[i].graph->device != device_)
+      return fail(MGPM_ERR_INVALID, "query graph lives on another device");
+    if (q[i].pattern < MGPM_TC || q[i].pattern > MGPM_P8)
+      return fail(MGPM_ERR_PATTERN, "Not support.");
+  }
+  // Checked before anything is allocated or built.
+  if (growth(q, nq) > max_bytes)
+    return fail(MGPM_ERR_CAPACITY, "batch workspace exceeds the HBM budget");
   queries_.assign(static_cast<size_t>(nq), Planned{});
+  any_census_ = false;
   for (int i = 0; i < nq; ++i) {
     mgpm_dgraph *g = q[i].graph;
     const int p = q[i].pattern;
-    if (!g) return fail(MGPM_ERR_INVALID, "query without a graph");
-    if (g->device != device_)
-      return fail(MGPM_ERR_INVALID, "query graph lives on another device");
-    if (p < MGPM_TC || p > MGPM_P8) return fail(MGPM_ERR_PATTERN, "Not support.");
-    if (p != MGPM_TC) {
+    Planned pq{g, p, 0, false};
+    if (p == MGPM_TC) {
+      pq.cost = query_cost(g, p);
+    } else if (census_) {
+      pq.census = true;
+      pq.cost = census_cost(g, p);
+      any_census_ = true;
+    } else {
       const int rc = pattern_make_resident(g, nullptr, p);
       if (rc) return rc;
+      pq.cost = query_cost(g, p);
     }
-    queries_[i] = Planned{g, p, query_cost(g, p)};
+    queries_[i] = pq;
   }
-  // Pattern-heavy batches run on at most kPatternLanes lanes: their kernels
-  // are throughput-bound with large working sets, and more concurrent ones
-  // measured slower (TC + P2 + P4 on 512 graphs: 4 lanes 192 queries/s,
-  // 8 lanes 170, 16 lanes 151); triangle counts gain from every lane.
+  // Pattern-heavy batches of per-edge kernels run on at most kPatternLanes
+  // lanes: their kernels are throughput-bound with large working sets, and
+  // more concurrent ones measured slower (TC + P2 + P4 on 512 graphs: 4
+  // lanes 192 queries/s, 8 lanes 170, 16 lanes 151); triangle counts and
+  // census queries gain from every lane.
   uint64_t pat_cost = 0, tc_cost = 0;
   for (const Planned &pq : queries_)
-    (pq.pattern == MGPM_TC ? tc_cost : pat_cost) += pq.cost;
+    (pq.pattern == MGPM_TC || pq.census ? tc_cost : pat_cost) += pq.cost;
   const int use = pat_cost > tc_cost ? std::min(L, kPatternLanes) : L;
   // Longest processing time first, each query to the least-loaded lane.
   std::vector<int> idx(static_cast<size_t>(nq));
@@ -109,43 +205,38 @@ int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
         std::min_element(load.begin(), load.end()) - load.begin());
     load[l] += queries_[i].cost;
     order_[l].push_back(i);
-    if (queries_[i].pattern != MGPM_TC)
+    if (queries_[i].pattern != MGPM_TC && !queries_[i].census)
       need_cap[l] = std::max<int64_t>(need_cap[l], queries_[i].g->pat.cap);
   }
-  // Workspace sizes; grow-only, checked against the limit first.
-  std::vector<int> grid(static_cast<size_t>(L), 0);
-  std::vector<uint64_t> want(static_cast<size_t>(L), 0);
-  uint64_t total = std::max<uint64_t>(scratch_.bytes, 32ull * uint64_t(nq));
+  // Census queries: the graph's DAG (built once, kept with the graph) and
+  // the lane's workspace, grown to the largest query on the lane.
+  for (int l = 0; l < L; ++l)
+    for (int i : order_[l])
+      if (queries_[i].census) {
+        const int rc = census_prepare(queries_[i].g, queries_[i].pattern,
+                                      census_work_[l], lanes_[l]);
+        if (rc) return rc;
+      }
   for (int l = 0; l < L; ++l) {
-    if (need_cap[l] == 0 ||
-        (slab_[l].p && slab_cap_[l] >= need_cap[l])) {
-      total += slab_[l].bytes;
+    if (need_cap[l] == 0 || (slab_[l].p && slab_cap_[l] >= need_cap[l]))
       continue;
-    }
     const uint64_t per_block = slab_bytes(need_cap[l], 1);
-    grid[l] = static_cast<int>(std::max<uint64_t>(
+    const int grid = static_cast<int>(std::max<uint64_t>(
         1, std::min<uint64_t>(uint64_t(cus_) * 4, kLaneSlabBudget / per_block)));
-    want[l] = per_block * uint64_t(grid[l]);
-    total += want[l];
-  }
-  if (total > max_bytes)
-    return fail(MGPM_ERR_CAPACITY, "batch workspace exceeds the HBM budget");
-  for (int l = 0; l < L; ++l) {
-    if (!want[l]) continue;
-    MGPM_HIP(slab_[l].alloc(want[l]));
+    MGPM_HIP(slab_[l].alloc(per_block * uint64_t(grid)));
     slab_cap_[l] = need_cap[l];
-    slab_grid_[l] = grid[l];
+    slab_grid_[l] = grid;
   }
-  if (scratch_.bytes < 32ull * uint64_t(nq))
-    MGPM_HIP(scratch_.alloc(32ull * uint64_t(std::max(nq, 64))));
+  const size_t per_query = kQueryHead + kQueryAcc;
+  if (scratch_.bytes < per_query * size_t(nq))
+    MGPM_HIP(scratch_.alloc(per_query * size_t(std::max(nq, 64))));
   if (pinned_n_ < static_cast<size_t>(nq)) {
     if (pinned_) (void)hipHostFree(pinned_);
     pinned_ = nullptr;
     pinned_n_ = 0;
     const size_t n = static_cast<size_t>(std::max(nq, 64));
     MGPM_HIP(hipHostMalloc(reinterpret_cast<void **>(&pinned_),
-                           2 * n * sizeof(unsigned long long),
-                           hipHostMallocDefault));
+                           per_query * n, hipHostMallocDefault));
     pinned_n_ = n;
   }
   return MGPM_OK;
@@ -156,7 +247,9 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
   if (nq == 0) return MGPM_OK;
   auto *counters = scratch_.as<unsigned long long>();
   auto *queues = reinterpret_cast<unsigned *>(counters + 2 * nq);
-  if (device_zero(scratch_.p, 8 * static_cast<size_t>(nq), s))
+  auto *accs = reinterpret_cast<unsigned long long *>(scratch_.as<char>() +
+                                                      kQueryHead * nq);
+  if (device_zero(scratch_.p, kQueryHead / 4 * static_cast<size_t>(nq), s))
     return fail(MGPM_ERR_HIP, "batch reset launch failed");
   MGPM_HIP(hipEventRecord(fork_, s));
   for (int l = 0; l < lanes(); ++l) {
@@ -177,6 +270,14 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
         tl.reset = false;
         if (tc_device_launch(p, tl, ls))
           return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
+      } else if (qp.census) {
+        // The lane's queries run in order on its one census workspace; the
+        // accumulators are copied out before the next query reuses it.
+        int rc = census_enqueue(qp.g, qp.pattern, census_work_[l], ls);
+        if (rc) return rc;
+        MGPM_HIP(hipMemcpyAsync(accs + kCensusSlots * size_t(i),
+                                census_acc(census_work_[l]), kQueryAcc,
+                                hipMemcpyDeviceToDevice, ls));
       } else {
         PatternDevicePlan p = pattern_device_plan(qp.g, qp.pattern);
         p.slab = slab_[l].as<int32_t>();
@@ -198,18 +299,29 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
 int BatchExec::enqueue_copy(hipStream_t s) {
   const int nq = size();
   if (nq == 0) return MGPM_OK;
-  MGPM_HIP(hipMemcpyAsync(pinned_, scratch_.p,
-                          2 * static_cast<size_t>(nq) * sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
+  // Counters, and the census accumulators when any query has them.
+  const size_t bytes =
+      (any_census_ ? kQueryHead + kQueryAcc : kQueryHead) * size_t(nq);
+  MGPM_HIP(hipMemcpyAsync(pinned_, scratch_.p, bytes, hipMemcpyDeviceToHost,
+                          s));
   return MGPM_OK;
 }
 
-uint64_t BatchExec::count(int i) const {
+int BatchExec::result(int i, uint64_t *count) const {
+  const Planned &qp = queries_[i];
+  const int nq = size();
+  if (qp.census) {
+    const unsigned long long *acc = reinterpret_cast<const unsigned long long *>(
+        reinterpret_cast<const char *>(pinned_) + kQueryHead * nq +
+        kQueryAcc * size_t(i));
+    return census_combine(qp.pattern, acc, count);
+  }
   const uint64_t c1 = pinned_[2 * i], c2 = pinned_[2 * i + 1];
-  const int p = queries_[i].pattern;
   // sm_multigpu.cu:209-232 on one rank: P4 and P6 report counters[1], the
   // other patterns counters[1] - counters[2]; TC is the single counter.
-  return (p == MGPM_TC || p == MGPM_P4 || p == MGPM_P6) ? c1 : c1 - c2;
+  const int p = qp.pattern;
+  *count = (p == MGPM_TC || p == MGPM_P4 || p == MGPM_P6) ? c1 : c1 - c2;
+  return MGPM_OK;
 }
 
 }  // namespace mgpm
@@ -285,7 +397,9 @@ int mgpm_batch_create(int device, const mgpm_batch_opts *opts,
     if (b->opts.lanes <= 0) b->opts.lanes = 4;
     MGPM_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount,
                                    device));
-    if ((rc = b->exec.init(device, b->cus, b->opts.lanes))) return rc;
+    if ((rc = b->exec.init(device, b->cus, b->opts.lanes,
+                           b->opts.pattern_census >= 0)))
+      return rc;
     MGPM_HIP(hipEventCreate(&b->ev0));
     MGPM_HIP(hipEventCreate(&b->ev1));
     if (b->opts.use_graph)
@@ -346,10 +460,14 @@ int mgpm_batch_run(mgpm_batch *b, mgpm_query *queries, int nq, void *stream,
     MGPM_HIP(hipStreamSynchronize(s));
     float kms = 0;
     MGPM_HIP(hipEventElapsedTime(&kms, b->ev0, b->ev1));
+    int worst = MGPM_OK;
     for (int i = 0; i < nq; ++i) {
-      queries[i].count = b->exec.count(i);
-      queries[i].status = MGPM_OK;
+      uint64_t c = 0;
+      queries[i].status = b->exec.result(i, &c);
+      queries[i].count = c;
+      if (queries[i].status) worst = queries[i].status;
     }
+    if (worst) return fail(worst, "batch: a census self-check failed");
     if (stats) {
       std::memset(stats, 0, sizeof(*stats));
       stats->kernel_ms = kms;

@@ -27,12 +27,28 @@ class BatchExec {
   ~BatchExec();
 
   // Creates the lane streams and events on `device` (current device).
-  int init(int device, int cus, int lanes);
+  // census: P1..P8 in closed form from the census (one census workspace
+  // per lane), else by the per-edge kernels (one slab per lane).
+  int init(int device, int cus, int lanes, bool census = true);
 
-  // Validates the queries, builds missing pattern plans, assigns queries to
-  // lanes and sizes the workspaces.  Fails with MGPM_ERR_CAPACITY, before
-  // allocating, when the workspaces would grow past `max_bytes` in total.
+  // Validates the queries, builds missing pattern plans (or the graphs'
+  // census DAGs), assigns queries to lanes and sizes the workspaces.  Fails
+  // with MGPM_ERR_CAPACITY, before allocating, when the workspaces would
+  // grow past `max_bytes` in total.
   int plan(const mgpm_query *q, int nq, uint64_t max_bytes);
+
+  // Device bytes plan() may add to the workspaces for these queries (a
+  // bound: every lane grown to the largest query's needs; 0 if they fit
+  // what is there).  query_need folds one query into the running maxima
+  // that growth_for turns into that bound, for callers growing a set.
+  uint64_t growth(const mgpm_query *q, int nq) const;
+  void query_need(const mgpm_query &q, uint64_t *want_census,
+                  uint64_t *want_slab) const;
+  uint64_t growth_for(uint64_t want_census, uint64_t want_slab, int nq) const;
+
+  // Frees the lanes' pattern workspaces (census and slabs) after the lanes
+  // drain; the next plan() grows them again.
+  void release();
 
   // Enqueues the planned queries on `s`: zero fill, fork to the lanes,
   // kernels, join back to `s`, copy of the counters to host memory.  The
@@ -42,8 +58,9 @@ class BatchExec {
   // Enqueues only the copy of the counters to host memory on `s`.
   int enqueue_copy(hipStream_t s);
 
-  // Combined count of planned query i (after the copy has completed).
-  uint64_t count(int i) const;
+  // Count of planned query i (after the copy has completed); returns
+  // MGPM_OK, or MGPM_ERR_INTERNAL if a census self-check failed.
+  int result(int i, uint64_t *count) const;
 
   int size() const { return static_cast<int>(queries_.size()); }
   int lanes() const { return static_cast<int>(lanes_.size()); }
@@ -54,7 +71,11 @@ class BatchExec {
     mgpm_dgraph *g = nullptr;
     int pattern = 0;
     uint64_t cost = 0;
+    bool census = false;
   };
+  bool census_ = true;
+  bool any_census_ = false;          // some planned query runs the census
+  std::unique_ptr<CensusWork[]> census_work_;  // per lane
   int device_ = 0;
   int cus_ = 0;
   std::vector<hipStream_t> lanes_;
@@ -65,7 +86,8 @@ class BatchExec {
   std::vector<int> slab_grid_;       // workgroups the slab holds
   std::vector<std::vector<int>> order_;  // per lane, query indices in order
   std::vector<Planned> queries_;
-  DevBuf scratch_;                   // 2 u64 counters + 4 u32 queues each
+  DevBuf scratch_;                   // per query: 2 u64 counters + 4 u32
+                                     // queues, then the census accumulators
   unsigned long long *pinned_ = nullptr;
   size_t pinned_n_ = 0;              // queries the pinned buffer holds
 };

@@ -404,7 +404,8 @@ int mgpm_dgraph_upload_ex(const mgpm_graph *g, int device, void *stream,
 int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
   if (!dg || !bytes) return fail(MGPM_ERR_INVALID, "null argument");
   *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->pat.bytes() +
-           dg->dag.bytes() + dg->scratch.bytes;
+           dg->dag.bytes() + dg->census.bytes() + dg->census_work.bytes() +
+           dg->scratch.bytes;
   return MGPM_OK;
 }
 

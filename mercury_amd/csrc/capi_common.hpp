@@ -135,6 +135,41 @@ struct DagResident {
   }
 };
 
+// The degree-oriented DAG the census counts over, built on the device on
+// the census's first call and kept with the graph, with the pair-table
+// layout of the 5-vertex census (a function of the graph alone).
+struct CensusGraph {
+  bool ready = false;
+  int32_t max_deg = 1;   // largest degree
+  int64_t max_out = 1;   // largest DAG out-degree
+  DevBuf deg, src, drp, dcol, dsrc;
+  bool pairs_ready = false;
+  uint64_t tab_total = 0;  // pair-table slots
+  DevBuf tab_off;          // n + 1 slot offsets
+  size_t bytes() const {
+    return deg.bytes + src.bytes + drp.bytes + dcol.bytes + dsrc.bytes +
+           tab_off.bytes;
+  }
+};
+
+// Census workspace of one caller (a graph's single-query path, a batch
+// lane, a server slot): grows to the largest query it has served and is
+// reused without allocating.
+struct CensusWork {
+  // Bytes the pair passes' per-workgroup arrays may take (more workgroups
+  // in flight on larger budgets): a graph's own single-query workspace
+  // takes 16 GiB, batch lanes and server slots less.
+  size_t budget = size_t(16) << 30;
+  DevBuf t, t_csr, tv, small, slab, wedges, touched, k5_slab;
+  DevBuf stamp, lo, mask, list, grows, anchors, table;
+  size_t bytes() const {
+    return t.bytes + t_csr.bytes + tv.bytes + small.bytes + slab.bytes +
+           wedges.bytes + touched.bytes + k5_slab.bytes + stamp.bytes +
+           lo.bytes + mask.bytes + list.bytes + grows.bytes + anchors.bytes +
+           table.bytes;
+  }
+};
+
 // Default task-cost chunk for a DAG of `arcs` arcs: 64K below 2^27 arcs,
 // else 64K times the largest power of two <= arcs / 2^26, at most 1M.  On
 // one MI355X, RMAT-22 and ER-22 (just under 2^26 arcs) count fastest with
@@ -168,10 +203,30 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
 inline int default_tc_items(uint64_t probes, uint64_t partners) {
   return probes < 32 * partners ? 4 : 8;
 }
-// P1, P4, P7 and P8 in closed form from a triangle, 4-cycle, 4-clique and
-// 5-clique census (census.cc).
+// P1..P8 in closed form from a subgraph census (census.cc): triangles,
+// 4-cycles, 4- and 5-cliques for P1, P4, P7 and P8, and the 5-vertex pair
+// passes for P2, P3, P5 and P6.
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms);
+// The census split for callers that run many queries: census_prepare
+// builds what the graph keeps (DAG, pair-table layout; may allocate and
+// synchronise `s`) and grows `w` for the pattern; census_enqueue then
+// enqueues the whole count on `s` with no allocation and no host
+// synchronisation, leaving the accumulators at census_acc(w);
+// census_combine turns them into the count (MGPM_ERR_INTERNAL if the
+// census's self-checks disagree).
+int census_prepare(mgpm_dgraph *dg, int pattern, CensusWork &w, hipStream_t s);
+// What census_prepare builds for the graph alone (DAG and pair-table
+// layout), with temporary buffers only.
+int census_graph_prepare(mgpm_dgraph *dg, hipStream_t s);
+int census_enqueue(mgpm_dgraph *dg, int pattern, CensusWork &w, hipStream_t s);
+unsigned long long *census_acc(const CensusWork &w);
+int census_combine(int pattern, const unsigned long long *acc,
+                   uint64_t *count);
+// Device bytes of a census workspace for this pattern on this graph
+// (before the graph's census DAG exists: a bound that includes it).
+uint64_t census_work_bytes(const mgpm_dgraph *dg, int pattern,
+                           size_t budget);
 // Rebuilds the resident triangle plan where it was built (host or device).
 int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s);
 // Builds pattern `pattern`'s plan if missing or if `split` (0 = the
@@ -197,6 +252,8 @@ struct mgpm_dgraph {
   mgpm::CsrResident csr;
   mgpm::PatternResident pat;
   mgpm::DagResident dag;
+  mgpm::CensusGraph census;            // DAG of the census
+  mgpm::CensusWork census_work;        // single-query census workspace
   mgpm::DevBuf scratch;               // queues + result counters
   unsigned long long *pinned = nullptr;  // host mirror of results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
   const int lane = lane64();
   const int64_t wid = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
   int32_t *slab = c.slab + wid * c.slab_len;
-  unsigned long long tri = 0, tdeg = 0, k4 = 0;
+  unsigned long long tri = 0, tdeg = 0, k4 = 0, crk = 0, bdeg = 0;
   for (;;) {
     uint32_t a = 0;
     if (lane == 0) a = atomicAdd(c.cursor, 1u);
@@ -88,7 +88,15 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
           hit = true;
           atomicAdd(&c.t[sb + p], 1u);
           atomicAdd(&c.t[lb + j], 1u);
-          tdeg += static_cast<unsigned long long>(c.deg[z]);
+          const unsigned long long dz = static_cast<unsigned long long>(c.deg[z]);
+          tdeg += dz;
+          if (c.full5) {
+            // Degrees in a triangle are >= 2: every factor is >= 0.
+            const unsigned long long ex = c.deg[x] - 2, ey = c.deg[y] - 2,
+                                     ez = dz - 2;
+            crk += choose2(ex) + choose2(ey) + choose2(ez);
+            bdeg += ex * ey + ex * ez + ey * ez;
+          }
         }
       }
       const unsigned long long mask = __ballot(hit);
@@ -137,39 +145,63 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
   tri = wave_sum_u64(tri);
   tdeg = wave_sum_u64(tdeg);
   k4 = wave_sum_u64(k4);
+  crk = wave_sum_u64(crk);
+  bdeg = wave_sum_u64(bdeg);
   if (lane == 0) {
     if (tri) atomicAdd(&c.acc[kCensusTriangles], tri);
     if (tdeg) atomicAdd(&c.acc[kCensusTriDegrees], tdeg);
     if (k4) atomicAdd(&c.acc[kCensusK4], k4);
+    if (crk) atomicAdd(&c.acc[kCensusCricket], crk);
+    if (bdeg) atomicAdd(&c.acc[kCensusBullDeg], bdeg);
   }
 }
 
 // Sums over vertices (C(d,3)) and over DAG arcs (C(t,2) and
 // (d_x - t - 1)(d_y - t - 1)).
+// With full5 also C(d,4), the fork and diamond-pendant sums per edge, and
+// twice the triangles per vertex into tv.
 __global__ void census_sums(CensusArgs c) {
-  unsigned long long s3 = 0, s2 = 0, sp = 0, st = 0;
+  unsigned long long s3 = 0, s2 = 0, sp = 0, st = 0, s4 = 0, fk = 0, dp3 = 0;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < c.n;
        v += int64_t(gridDim.x) * blockDim.x) {
     const unsigned long long d = static_cast<unsigned long long>(c.deg[v]);
     s3 += choose3(d);
+    if (c.full5) s4 += choose4(d);
+    unsigned long long tsum = 0;
     for (int64_t a = c.drp[v]; a < c.drp[v + 1]; ++a) {
       const unsigned long long t = c.t[a];
-      const unsigned long long dy =
-          static_cast<unsigned long long>(c.deg[c.dcol[a]]);
+      const int32_t y = c.dcol[a];
+      const unsigned long long dy = static_cast<unsigned long long>(c.deg[y]);
       s2 += choose2(t);
       st += choose3(t);
       sp += (d - t - 1) * (dy - t - 1);
+      if (c.full5) {
+        // d, dy >= 1; t > 0 only when both are >= 2, so t (d - 2) is 0
+        // whenever d - 2 would wrap.
+        fk += choose2(d - 1) * (dy - 1) - t * (d - 2) +
+              choose2(dy - 1) * (d - 1) - t * (dy - 2);
+        dp3 += choose2(t) * (d + dy - 6);
+        tsum += t;
+        if (t) atomicAdd(&c.tv[y], t);
+      }
     }
+    if (c.full5 && tsum) atomicAdd(&c.tv[v], tsum);
   }
   s3 = wave_sum_u64(s3);
   s2 = wave_sum_u64(s2);
   sp = wave_sum_u64(sp);
   st = wave_sum_u64(st);
+  s4 = wave_sum_u64(s4);
+  fk = wave_sum_u64(fk);
+  dp3 = wave_sum_u64(dp3);
   if (lane64() == 0) {
     if (s3) atomicAdd(&c.acc[kCensusDeg3], s3);
     if (s2) atomicAdd(&c.acc[kCensusEdgePairs], s2);
     if (sp) atomicAdd(&c.acc[kCensusPathPairs], sp);
     if (st) atomicAdd(&c.acc[kCensusEdgeTriples], st);
+    if (s4) atomicAdd(&c.acc[kCensusDeg4], s4);
+    if (fk) atomicAdd(&c.acc[kCensusFork], fk);
+    if (dp3) atomicAdd(&c.acc[kCensusDp3], dp3);
   }
 }
 
@@ -207,7 +239,8 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
   int32_t *F = Cz + 4 * L;
   unsigned long long *rows = reinterpret_cast<unsigned long long *>(
       Cz + ((4 * L + c.max_deg + 1) & ~int64_t(1)));
-  unsigned long long kk = 0, tek = 0, tdk = 0, k4 = 0, k5 = 0, bad = 0;
+  unsigned long long kk = 0, tek = 0, tdk = 0, k4 = 0, k5 = 0, bad = 0,
+                     dp2 = 0, gem = 0;
   for (;;) {
     uint32_t a = 0;
     if (lane == 0) a = atomicAdd(c.cursor + 2, 1u);
@@ -268,6 +301,19 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
     }
     if (lane == 0 && static_cast<uint32_t>(te) != ta) ++bad;
     wave_sync();  // C and F written by other lanes
+    if (c.full5) {
+      // Per triangle {x, y, z} (each seen once, at its lowest arc): the
+      // diamond-pendant and gem terms from its three edge counts (>= 1)
+      // and degrees (>= 2).
+      for (int i = lane; i < cnt; i += 64) {
+        const unsigned long long txy = ta - 1, txz = c.t[Cxz[i]] - 1,
+                                 tyz = c.t[Cyz[i]] - 1;
+        const unsigned long long ez = c.deg[Cz[i]] - 2;
+        dp2 += txy * ez + txz * static_cast<unsigned long long>(dy - 2) +
+               tyz * static_cast<unsigned long long>(dx - 2);
+        gem += txy * txz + txy * tyz + txz * tyz;
+      }
+    }
     const unsigned long long dxy = static_cast<unsigned long long>(dx + dy);
     if (te <= kSerialKt) {
       // 64 triangles at a time; their membership tests (the shorter of F
@@ -416,7 +462,11 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
   k4 = wave_sum_u64(k4);
   k5 = wave_sum_u64(k5);
   bad = wave_sum_u64(bad);
+  dp2 = wave_sum_u64(dp2);
+  gem = wave_sum_u64(gem);
   if (lane == 0) {
+    if (dp2) atomicAdd(&c.acc[kCensusDp2], dp2);
+    if (gem) atomicAdd(&c.acc[kCensusGem], gem);
     if (kk) atomicAdd(&c.acc[kCensusTriPairs], kk);
     if (tek) atomicAdd(&c.acc[kCensusTriEdgeK], tek);
     if (tdk) atomicAdd(&c.acc[kCensusTriDegK], tdk);
@@ -445,10 +495,11 @@ __global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
   uint32_t *cnt = c.wedges + size_t(blockIdx.x) * size_t(c.n);
   int32_t *touched = c.touched + size_t(blockIdx.x) * size_t(c.n);
   __shared__ unsigned s_v, s_nt;
-  __shared__ unsigned long long s_acc[kC4Threads / 64];
+  __shared__ unsigned long long s_acc[kC4Threads / 64],
+      s_ban[kC4Threads / 64], s_tq[kC4Threads / 64];
   const int lane = lane64(), wv = threadIdx.x >> 6;
   constexpr int nwv = kC4Threads / 64;
-  unsigned long long acc = 0;
+  unsigned long long acc = 0, ban = 0, tq = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       s_v = atomicAdd(c.cursor + 1, 1u);
@@ -469,21 +520,61 @@ __global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
       }
     }
     __syncthreads();
+    if (c.full5) {
+      // Second walk over the same wedges with the final counts: the wedge
+      // v-u-w lies on cnt[w] - 1 cycles of the pair (v, w), and each puts
+      // its edges vu, uw and its middle u on them (4-cycles per edge and
+      // per vertex, weighted by t_e and d_u - 2).
+      for (int64_t i = c.rowptr[v] + wv; i < ev; i += nwv) {
+        const int32_t u = c.col[i];
+        if (!rank_below(c.deg, u, static_cast<int32_t>(v))) continue;
+        const unsigned long long tvu = c.t_csr[i];
+        const unsigned long long du2 = c.deg[u] - 2;
+        const int64_t eu = c.rowptr[u + 1];
+        for (int64_t j = c.rowptr[u] + lane; j < eu; j += 64) {
+          const int32_t w = c.col[j];
+          if (!rank_below(c.deg, w, static_cast<int32_t>(v))) continue;
+          // The counts were made by atomics (in L2): read them past L1.
+          const unsigned long long x =
+              __hip_atomic_load(&cnt[w], __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT) - 1u;
+          ban += x * du2;
+          tq += x * (tvu + c.t_csr[j]);
+        }
+      }
+      __syncthreads();  // every count read before the clear below
+    }
     const unsigned nt = s_nt;
+    const unsigned long long dv = c.deg[v];
     for (unsigned k = threadIdx.x; k < nt; k += kC4Threads) {
       // Read and clear in one atomic (the counts were made by atomics).
-      const unsigned long long x = atomicExch(&cnt[touched[k]], 0u);
-      acc += x * (x - 1) / 2;
+      const int32_t w = touched[k];
+      const unsigned long long x = atomicExch(&cnt[w], 0u);
+      const unsigned long long cyc = choose2(x);
+      acc += cyc;
+      if (c.full5) ban += cyc * (dv + c.deg[w] - 4);
     }
     __syncthreads();  // s_v, s_nt and the list are reused
   }
   acc = wave_sum_u64(acc);
-  if (lane == 0) s_acc[wv] = acc;
+  ban = wave_sum_u64(ban);
+  tq = wave_sum_u64(tq);
+  if (lane == 0) {
+    s_acc[wv] = acc;
+    s_ban[wv] = ban;
+    s_tq[wv] = tq;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int k = 0; k < nwv; ++k) t += s_acc[k];
+    unsigned long long t = 0, b = 0, q = 0;
+    for (int k = 0; k < nwv; ++k) {
+      t += s_acc[k];
+      b += s_ban[k];
+      q += s_tq[k];
+    }
     if (t) atomicAdd(&c.acc[kCensusC4], t);
+    if (b) atomicAdd(&c.acc[kCensusBanner], b);
+    if (q) atomicAdd(&c.acc[kCensusCycEdgeT], q);
   }
 }
 
@@ -509,11 +600,17 @@ int census_k5_launch(const CensusArgs &c, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int census_dag_sources(const int64_t *drp, int32_t n, int32_t *dsrc, int cus,
+                       hipStream_t s) {
+  const int gv = static_cast<int>(std::max<int64_t>(
+      1, std::min<int64_t>((int64_t(n) + 255) / 256, int64_t(cus) * 16)));
+  hipLaunchKernelGGL(dag_sources, dim3(gv), dim3(256), 0, s, drp, n, dsrc);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int census_launch(const CensusArgs &c, int cus, hipStream_t s) {
   const int gv = static_cast<int>(std::max<int64_t>(
       1, std::min<int64_t>((int64_t(c.n) + 255) / 256, int64_t(cus) * 16)));
-  hipLaunchKernelGGL(dag_sources, dim3(gv), dim3(256), 0, s, c.drp, c.n,
-                     c.dsrc);
   const int blocks = census_waves(cus) / kWaves;
   hipLaunchKernelGGL(census_triangles, dim3(blocks), dim3(kWaves * 64), 0, s,
                      c);

@@ -108,9 +108,7 @@ extern "C" int mgpm_pattern_count(mgpm_dgraph *dg, int pattern,
     int rc = set_device(dg->device);
     if (rc) return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if ((pattern == MGPM_P1 || pattern == MGPM_P4 || pattern == MGPM_P7 ||
-         pattern == MGPM_P8) &&
-        !(tune && tune->pattern_census < 0)) {
+    if (!(tune && tune->pattern_census < 0)) {
       float kms = 0;
       if ((rc = census_pattern_count(dg, pattern, s, count, &kms))) return rc;
       if (stats) {

@@ -61,18 +61,27 @@ struct mgpm_server {
 
 namespace {
 
-// Launches the pending requests `tickets` on slot `sl` (server lock held).
+void finish(mgpm_server *s, uint64_t t, uint64_t count, int status,
+            int batch, Clock::time_point now);
+
+mgpm_query query_of(const mgpm_server *s, const Req &r) {
+  mgpm_query q{};
+  q.graph = s->graphs[r.graph];
+  q.pattern = r.pattern;
+  return q;
+}
+
+// Launches the requests `tickets` on slot `sl` (server lock held); their
+// workspace growth was checked against the budget by the caller.
 int launch_slot(mgpm_server *s, Slot &sl, const std::vector<uint64_t> &tickets) {
   std::vector<mgpm_query> q(tickets.size());
-  for (size_t i = 0; i < tickets.size(); ++i) {
-    const Req &r = s->reqs[tickets[i]];
-    q[i].graph = s->graphs[r.graph];
-    q[i].pattern = r.pattern;
-  }
-  // The slot's workspace may grow up to what the budget leaves.
+  for (size_t i = 0; i < tickets.size(); ++i)
+    q[i] = query_of(s, s->reqs[tickets[i]]);
   const uint64_t before = sl.exec.footprint();
   const uint64_t others = s->resident + s->workspace - before;
-  const uint64_t room = s->budget > others ? s->budget - others : 0;
+  const uint64_t room = s->budget > others + before
+                            ? s->budget - others - before
+                            : 0;
   int rc = sl.exec.plan(q.data(), static_cast<int>(q.size()), room);
   s->workspace = s->workspace - before + sl.exec.footprint();
   if (rc) return rc;
@@ -83,6 +92,56 @@ int launch_slot(mgpm_server *s, Slot &sl, const std::vector<uint64_t> &tickets) 
   if (s->inflight++ == 0) s->busy_since = Clock::now();
   s->st.launches++;
   return MGPM_OK;
+}
+
+// Budget left for slot `sl`'s workspace to grow into.
+uint64_t room_for(const mgpm_server *s) {
+  const uint64_t used = s->resident + s->workspace;
+  return s->budget > used ? s->budget - used : 0;
+}
+
+// Takes pending requests in arrival order for free slot `sl` while their
+// workspaces fit the budget (server lock held).  The first request that
+// does not fit stops the intake and waits: when no batch is in flight, the
+// idle slots' workspaces are released for it, and if it still cannot fit
+// it fails alone (MGPM_ERR_CAPACITY) while the rest keep their turn.
+std::vector<uint64_t> admit(mgpm_server *s, Slot &sl) {
+  std::vector<uint64_t> take;
+  uint64_t wc = 0, ws = 0;
+  const size_t cap = static_cast<size_t>(s->cfg.max_batch);
+  while (!s->pending.empty() && take.size() < cap) {
+    const uint64_t t = s->pending.front();
+    const mgpm_query q = query_of(s, s->reqs[t]);
+    uint64_t c2 = wc, s2 = ws;
+    sl.exec.query_need(q, &c2, &s2);
+    const int nq = static_cast<int>(take.size()) + 1;
+    if (sl.exec.growth_for(c2, s2, nq) <= room_for(s)) {
+      wc = c2;
+      ws = s2;
+      take.push_back(t);
+      s->pending.pop_front();
+      continue;
+    }
+    if (!take.empty() || s->inflight > 0) {  // wait for memory to free
+      s->st.deferred++;
+      break;
+    }
+    // Nothing in flight: free every idle slot's workspace and try again.
+    uint64_t freed = 0;
+    for (Slot &o : s->slots) {
+      if (o.busy) continue;
+      const uint64_t b = o.exec.footprint();
+      o.exec.release();
+      freed += b - o.exec.footprint();
+    }
+    s->workspace -= std::min(s->workspace, freed);
+    if (sl.exec.growth_for(c2, s2, nq) <= room_for(s)) continue;
+    s->pending.pop_front();  // can never fit: fails alone
+    s->st.deferred_failed++;
+    finish(s, t, 0, MGPM_ERR_CAPACITY, 0, Clock::now());
+    s->cv_done.notify_all();
+  }
+  return take;
 }
 
 void finish(mgpm_server *s, uint64_t t, uint64_t count, int status,
@@ -114,9 +173,11 @@ void scheduler(mgpm_server *s) {
       const int nb = static_cast<int>(sl.tickets.size());
       if (q != hipSuccess) set_error(std::string("server batch: ") +
                                      hipGetErrorString(q));
-      for (int i = 0; i < nb; ++i)
-        finish(s, sl.tickets[i], q == hipSuccess ? sl.exec.count(i) : 0,
-               q == hipSuccess ? MGPM_OK : MGPM_ERR_HIP, nb, now);
+      for (int i = 0; i < nb; ++i) {
+        uint64_t c = 0;
+        const int st = q == hipSuccess ? sl.exec.result(i, &c) : MGPM_ERR_HIP;
+        finish(s, sl.tickets[i], c, st, nb, now);
+      }
       sl.tickets.clear();
       sl.busy = false;
       if (--s->inflight == 0)
@@ -126,16 +187,12 @@ void scheduler(mgpm_server *s) {
       progressed = true;
     }
     if (progressed) s->cv_done.notify_all();
-    // Admissions: every free slot takes the pending requests, FIFO.
+    // Admissions: every free slot takes pending requests, FIFO, as far as
+    // the memory budget allows.
     for (Slot &sl : s->slots) {
       if (sl.busy || s->pending.empty()) continue;
-      std::vector<uint64_t> take;
-      const size_t k = std::min<size_t>(s->pending.size(),
-                                        static_cast<size_t>(s->cfg.max_batch));
-      for (size_t i = 0; i < k; ++i) {
-        take.push_back(s->pending.front());
-        s->pending.pop_front();
-      }
+      std::vector<uint64_t> take = admit(s, sl);
+      if (take.empty()) continue;
       const int rc = launch_slot(s, sl, take);
       if (rc) {  // isolate the failure to these requests; keep serving
         const auto now = Clock::now();
@@ -179,7 +236,9 @@ int mgpm_server_create(const mgpm_server_config *cfg, mgpm_server **out) {
     for (Slot &sl : s->slots) {
       MGPM_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
       MGPM_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-      if ((rc = sl.exec.init(cfg->device, s->cus, s->cfg.lanes))) return rc;
+      if ((rc = sl.exec.init(cfg->device, s->cus, s->cfg.lanes,
+                             s->cfg.pattern_census >= 0)))
+        return rc;
     }
     s->st.budget_bytes = s->budget;
     s->sched = std::thread(scheduler, s.get());
@@ -198,15 +257,21 @@ int mgpm_server_add_graph(mgpm_server *s, const mgpm_graph *g,
   try {
     int rc = set_device(s->cfg.device);
     if (rc) return rc;
-    // Upload (TC plan) and build every pattern's plan, then admit by
-    // footprint (slots hold the pattern scratch, so the graph holds none).
+    // Upload (TC plan) and build what the pattern counts keep with the
+    // graph -- the census DAG and pair-table layout, or every pattern's
+    // per-edge plan -- then admit by footprint (the slots hold the
+    // workspaces, so the graph holds none).
     mgpm_dgraph *dg = nullptr;
     if ((rc = mgpm_dgraph_upload(g, s->cfg.device, nullptr, &dg))) return rc;
-    for (int p = MGPM_P1; p <= MGPM_P8; ++p) {
-      if ((rc = pattern_make_resident(dg, nullptr, p))) {
-        mgpm_dgraph_free(dg);
-        return rc;
-      }
+    if (s->cfg.pattern_census >= 0) {
+      rc = census_graph_prepare(dg, nullptr);
+    } else {
+      for (int p = MGPM_P1; p <= MGPM_P8 && !rc; ++p)
+        rc = pattern_make_resident(dg, nullptr, p);
+    }
+    if (rc) {
+      mgpm_dgraph_free(dg);
+      return rc;
     }
     uint64_t bytes = 0;
     mgpm_dgraph_footprint(dg, &bytes);

@@ -53,13 +53,15 @@ class Query(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("use_graph", C.c_int32),
-                ("autotune", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("autotune", C.c_int32), ("pattern_census", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32),
                 ("max_batch", C.c_int32), ("lanes", C.c_int32),
-                ("hbm_budget", C.c_uint64), ("reserved", C.c_uint64 * 2)]
+                ("hbm_budget", C.c_uint64), ("pattern_census", C.c_int32),
+                ("reserved32", C.c_int32), ("reserved", C.c_uint64 * 1)]
 
 
 class Request(C.Structure):
@@ -74,7 +76,8 @@ class ServerStats(C.Structure):
                 ("failed", C.c_uint64), ("launches", C.c_uint64),
                 ("resident_bytes", C.c_uint64),
                 ("workspace_bytes", C.c_uint64),
-                ("budget_bytes", C.c_uint64), ("busy_ms", C.c_double)]
+                ("budget_bytes", C.c_uint64), ("busy_ms", C.c_double),
+                ("deferred", C.c_uint64), ("deferred_failed", C.c_uint64)]
 
 
 class GraphInfo(C.Structure):
@@ -400,10 +403,12 @@ class Batch:
     (``mgpm_batch``)."""
 
     def __init__(self, device: int = 0, lanes: int = 0,
-                 use_graph: bool = False, autotune: bool = False):
+                 use_graph: bool = False, autotune: bool = False,
+                 census: bool = True):
         self._h = C.c_void_p()
         opts = BatchOpts(lanes=lanes, use_graph=int(bool(use_graph)),
-                         autotune=int(bool(autotune)))
+                         autotune=int(bool(autotune)),
+                         pattern_census=0 if census else -1)
         check(lib.mgpm_batch_create(device, C.byref(opts), C.byref(self._h)),
               "mgpm_batch_create")
 
@@ -440,9 +445,10 @@ class Server:
     """Continuous-batching count server (``mgpm_server``)."""
 
     def __init__(self, device: int = 0, slots: int = 0, max_batch: int = 0,
-                 hbm_budget: int = 0, lanes: int = 0):
+                 hbm_budget: int = 0, lanes: int = 0, census: bool = True):
         cfg = ServerConfig(device=device, slots=slots, max_batch=max_batch,
-                           lanes=lanes, hbm_budget=hbm_budget)
+                           lanes=lanes, hbm_budget=hbm_budget,
+                           pattern_census=0 if census else -1)
         self._h = C.c_void_p()
         check(lib.mgpm_server_create(C.byref(cfg), C.byref(self._h)),
               "mgpm_server_create")

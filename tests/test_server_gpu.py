@@ -108,3 +108,50 @@ def test_server_rejects_bad_requests(gpu_available):
         srv.wait(123456)
     assert ei.value.code == -11
     srv.close()
+
+
+def _lane_bytes(g, p):
+    """Workspace one census query takes on one lane (a one-lane batch)."""
+    b = ma.Batch(0, lanes=1)
+    dg = ma.DeviceGraph(g, 0)
+    b.run([(dg, p)])
+    return b.footprint()
+
+
+def test_server_waits_for_memory_and_fails_oversized_alone(gpu_available):
+    """A budget that holds one small census workspace at a time: requests
+    wait for memory to free instead of failing (the stats count the
+    waits), every one of them completes with its single-query count, and a
+    request whose workspace cannot fit even with every idle workspace
+    released fails alone with MGPM_ERR_CAPACITY."""
+    small = [ma.Graph.rmat(8, 8, seed=90 + i) for i in range(4)]
+    big = ma.Graph.rmat(14, 16, seed=95)
+    w_small = max(_lane_bytes(g, 2) for g in small)
+    w_big = _lane_bytes(big, 2)
+    assert w_big > 3 * w_small
+    probe = ma.Server(0, slots=2, lanes=1)
+    for g in small + [big]:
+        probe.add_graph(g)
+    resident = probe.stats().resident_bytes
+    probe.close()
+    srv = ma.Server(0, slots=2, lanes=1,
+                    hbm_budget=resident + (3 * w_small) // 2)
+    ids = [srv.add_graph(g) for g in small + [big]]
+    alone = {(i, p): ma.DeviceGraph(g, 0).pattern_count(p)[0]
+             for i, g in enumerate(small) for p in (0, 2, 5)}
+    reqs = [(i % 4, (0, 2, 5)[i % 3]) for i in range(24)]
+    tickets = [srv.submit(ids[g], p) for g, p in reqs[:12]]
+    big_ticket = srv.submit(ids[4], 2)
+    tickets += [srv.submit(ids[g], p) for g, p in reqs[12:]]
+    for t, (g, p) in zip(tickets, reqs):
+        r = srv.wait(t)
+        assert r.status == 0, (g, p)
+        assert r.count == alone[(g, p)], (g, p)
+    with pytest.raises(ma.MgpmError) as ei:
+        srv.wait(big_ticket)
+    assert ei.value.code == -9
+    st = srv.stats()
+    assert st.completed == 24 and st.failed == 1
+    assert st.deferred_failed == 1 and st.deferred >= 1
+    assert st.resident_bytes + st.workspace_bytes <= st.budget_bytes
+    srv.close()
