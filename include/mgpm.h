@@ -230,6 +230,26 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
 int mgpm_pattern_count(mgpm_dgraph *dg, int pattern, const mgpm_tune *tune,
                        void *stream, uint64_t *count, mgpm_stats *stats);
 
+/* ---- allocation-free count --------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the count of mgpm_tc_count (MGPM_TC)
+ * or of mgpm_pattern_count's per-edge kernels (P1..P8; the same value the
+ * census gives) with every scratch buffer -- queue heads, counters and the
+ * per-wave frontier slab -- in one fixed library array of
+ * MGPM_NOALLOC_SCRATCH_BYTES per GPU, held from first use until the
+ * process ends.  No hipMalloc, hipFree or allocating library call; the
+ * pattern kernels run on as many workgroups as the array holds slabs for.
+ * Needs the pattern's plan resident (one earlier mgpm_pattern_count of
+ * that pattern with pattern_census = -1 builds it): MGPM_ERR_INVALID
+ * otherwise.  MGPM_ERR_CAPACITY if one workgroup's slab (4 waves x 8 x
+ * max degree x 4 bytes) exceeds the array, i.e. max degree above 524 286.
+ * Synchronises `stream`.  Calls on one GPU share its array: calls that
+ * overlap on one GPU race. */
+#define MGPM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mgpm_pattern_count_noalloc(mgpm_dgraph *dg, int pattern, void *stream,
+                               uint64_t *count);
+
 /* ---- autotuner --------------------------------------------------------------
  * No counterpart in the reference, whose schedule constants are fixed
  * (tuning_schedules.cuh:41-47, tc_challenge.cu:67-73).  MGPM_TC:

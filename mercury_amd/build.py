@@ -40,7 +40,7 @@ LIB_SOURCES = [
     "patterns.cc", "batch.cc", "autotune.cc", "server.cc", "dag.cc",
     "tc_dplan.cc", "census.cc", "tc_kernels.hip", "pattern_kernels.hip",
     "dag_kernels.hip", "tc_dplan_kernels.hip", "census_kernels.hip",
-    "census5_kernels.hip",
+    "census5_kernels.hip", "noalloc.hip",
 ]
 CLI_SOURCES = {"tc_challenge": "cli_tc_challenge.cc", "sm": "cli_sm.cc",
                "convert": "cli_convert.cc"}

@@ -121,6 +121,8 @@ def _load() -> C.CDLL:
                                     C.POINTER(Stats)]),
         "mgpm_pattern_count": (C.c_int, [P, C.c_int, C.POINTER(Tune), P,
                                          C.POINTER(u64), C.POINTER(Stats)]),
+        "mgpm_pattern_count_noalloc": (C.c_int, [P, C.c_int, P,
+                                                 C.POINTER(u64)]),
         "mgpm_dgraph_orientation": (C.c_int, [P, i32, P, P, P, P, P, P, P, P,
                                               C.POINTER(Stats)]),
         "mgpm_autotune": (C.c_int, [P, C.c_int, i32, C.POINTER(Tune),
@@ -158,6 +160,7 @@ EXPORTED = [
     "mgpm_dgraph_upload", "mgpm_dgraph_upload_ex", "mgpm_dgraph_footprint",
     "mgpm_dgraph_free",
     "mgpm_tc_count", "mgpm_pattern_count", "mgpm_autotune",
+    "mgpm_pattern_count_noalloc",
     "mgpm_dgraph_orientation",
     "mgpm_batch_create",
     "mgpm_batch_run", "mgpm_batch_footprint", "mgpm_batch_free",
@@ -348,6 +351,17 @@ class DeviceGraph:
                                      C.c_void_p(stream or 0), C.byref(cnt),
                                      C.byref(st)), "mgpm_pattern_count")
         return cnt.value, st
+
+    def pattern_count_noalloc(self, pattern: int,
+                              stream: Optional[int] = None) -> int:
+        """mgpm_pattern_count_noalloc: the count with every scratch buffer
+        in the library's fixed per-GPU array (no device allocation)."""
+        cnt = C.c_uint64()
+        check(lib.mgpm_pattern_count_noalloc(self._h, int(pattern),
+                                             C.c_void_p(stream or 0),
+                                             C.byref(cnt)),
+              "mgpm_pattern_count_noalloc")
+        return cnt.value
 
     def orientation(self, deg_thd: int = 128,
                     stream: Optional[int] = None) -> tuple[dict, Stats]:
