@@ -40,6 +40,11 @@ WORKLOADS = {
 }
 
 
+def log(msg: str) -> None:
+    """A progress line on stderr (the JSON result goes to stdout)."""
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
 def make_graph(ma, kind, scale, ef, seed):
     if kind == "rmat":
         return ma.Graph.rmat(scale, ef, seed=seed)
@@ -102,7 +107,49 @@ def cpu_baseline(g, count_gpu, m):
                    "orc_tc_dag), OpenMP dynamic over vertices, "
                    f"{dt:.2f} s, {cpu_model}"),
         "seconds": dt, "count": want, "count_matches_gpu": want == count_gpu,
+        "nproc": os.cpu_count(), "cpu_model": cpu_model,
     }
+
+
+def cpu_small_baselines(ma) -> dict:
+    """The other CPU baselines of the counting path, on the host cores of
+    this run: the triangle count of RMAT-16 (ef 16, seed 1) on one thread,
+    and the pattern oracle (the reference's nested set algebra, OpenMP
+    over vertices) for P1..P8 on RMAT-10 (ef 16, seed 4), the largest RMAT
+    scale where every pattern finishes in seconds, with its set-op work W
+    and the GPU census time of the same count."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    g = ma.Graph.rmat(16, 16, seed=1)
+    dag = g.orientation(128)
+    t0 = time.perf_counter()
+    want = oracle.tc_dag(dag["dag_rowptr"], dag["dag_colidx"], threads=1)
+    dt = time.perf_counter() - t0
+    dg = ma.DeviceGraph(g, 0)
+    gpu, st = dg.tc_count()
+    dg.close()
+    m = int(g.nnz) // 2
+    out = {"tc_rmat16_1thread": {
+        "seconds": dt, "edges_per_s": m / dt, "cores": 1, "count": want,
+        "gpu_kernel_ms": st.kernel_ms, "count_matches_gpu": gpu == want}}
+    g = ma.Graph.rmat(10, 16, seed=4)
+    rp, ci = g.csr()
+    dg = ma.DeviceGraph(g, 0)
+    rows = {}
+    for p in range(1, 9):
+        t0 = time.perf_counter()
+        c, w = oracle.pattern_work(rp, ci, p, threads=threads)
+        dt = time.perf_counter() - t0
+        dg.pattern_count(p)
+        gc, st = dg.pattern_count(p)
+        rows[f"P{p}"] = {"oracle_s": dt, "set_op_work": w,
+                         "gpu_census_ms": st.kernel_ms, "count": c,
+                         "count_matches_gpu": gc == c}
+    dg.close()
+    log("cpu baselines done")
+    out["patterns_rmat10"] = {"cores": threads, "graph": "RMAT scale 10, "
+                              "ef 16, seed 4", **rows}
+    return out
 
 
 def calibrate_copy(mib: int) -> int:
@@ -131,71 +178,106 @@ def batch_graphs(ma, count, lo, hi, seed=5):
                    for i, (s, ef) in enumerate(specs)]
 
 
-def bench_batch(ma, args) -> int:
+def run_batch(ma, args) -> dict:
+    """The batched workload: every graph's queries (default TC, P2, P4 on
+    512 RMAT graphs, scale uniform in [10, 18], edge factor in
+    {4, 8, 16, 32}, seed 5) in one batch call against one call per query,
+    each synchronising, with the same methods (triangle kernels; P1..P8 by
+    the census).  Counts must agree between the two, and with the oracle on
+    the graphs of scale <= 10."""
+    import oracle
     names = [p.strip() for p in args.batch_patterns.split(",")]
     pats = [ma.PATTERN_NAMES[p] for p in names]
     lo, hi = args.batch_scales
     t0 = time.perf_counter()
     specs, graphs = batch_graphs(ma, args.batch_graphs, lo, hi)
     dgs = [ma.DeviceGraph(g, 0) for g in graphs]
-    for dg in dgs:  # build pattern plans outside the timed region
-        if any(p != 0 for p in pats):
-            dg.pattern_count(4)
     prep_s = time.perf_counter() - t0
+    log(f"batch: {len(graphs)} graphs uploaded ({prep_s:.1f} s)")
     queries = [(dg, p) for dg in dgs for p in pats]
     sync = lambda: ma.native.check(ma.native.lib.mgpm_device_synchronize(0),
                                    "sync")
-    # one launch (and one synchronisation) per query, with the kernels the
-    # batch runs (P1, P4, P7, P8 by their per-edge kernels, not the census)
-    kern = ma.make_tune(pattern_census=-1)
-    one = lambda dg, p: dg.pattern_count(p, kern if p in (1, 4, 7, 8) else None)[0]
-    for _ in range(args.warmup):
-        single = [one(dg, p) for dg, p in queries]
+    med = lambda x: sorted(x)[len(x) // 2]
+    # one call (and one synchronisation) per query
+    one = lambda dg, p: dg.pattern_count(p)[0]
+
+    def all_single():
+        res = []
+        for k, (dg, p) in enumerate(queries):
+            res.append(one(dg, p))
+            if (k + 1) % 256 == 0:
+                log(f"batch: {k + 1}/{len(queries)} single queries done")
+        return res
+    for _ in range(max(1, args.batch_warmup)):
+        single = all_single()
     sync()
     t_single = []
-    for _ in range(args.steps):
+    for _ in range(args.batch_steps):
         t0 = time.perf_counter()
-        single = [one(dg, p) for dg, p in queries]
+        single = all_single()
         sync()
         t_single.append(time.perf_counter() - t0)
-    # the same queries in one batch call, overlapped over the lanes
+        log(f"batch: one call per query {t_single[-1]:.2f} s")
+    # the same queries in one batch call: one census per graph serves its
+    # pattern queries, lanes overlap the graphs
     b = ma.Batch(0, lanes=args.batch_lanes, use_graph=args.batch_use_graph)
-    for _ in range(max(1, args.warmup)):
+    for _ in range(max(1, args.batch_warmup)):
         counts, st = b.run(queries)
     sync()
+    log("batch: warm-up batch done")
     t_batch, kms = [], []
-    for _ in range(args.steps):
+    for _ in range(args.batch_steps):
         t0 = time.perf_counter()
         counts, st = b.run(queries)
         sync()
         t_batch.append(time.perf_counter() - t0)
         kms.append(st.kernel_ms)
+        log(f"batch: one batch {t_batch[-1]:.2f} s")
+    b.close()
+    # the oracle on the small graphs (its pattern algebra is cubic)
+    checked, oracle_ok = 0, True
+    for k, ((s_, ef), g) in enumerate(zip(specs, graphs)):
+        if s_ > 10 or ef > 16:
+            continue
+        rp, ci = g.csr()
+        for j, p in enumerate(pats):
+            want = oracle.tc(rp, ci) if p == 0 else oracle.pattern(rp, ci, p)[0]
+            oracle_ok &= counts[k * len(pats) + j] == want
+            checked += 1
     nq = len(queries)
-    med = lambda x: sorted(x)[len(x) // 2]
-    out = {
-        "metric": "batched queries/s",
-        "value": nq / med(t_batch),
-        "unit": "queries/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": med(t_batch) * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"batch: {args.batch_graphs} RMAT graphs, "
-                               f"scale uniform in [{lo},{hi}], edge factor "
-                               "uniform in {4,8,16,32}, seed 5; queries "
-                               f"{'+'.join(names)} per graph",
-                   "queries": nq, "lanes": args.batch_lanes,
-                   "hip_graph": bool(args.batch_use_graph),
-                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4")},
+    return {
+        "workload": f"{args.batch_graphs} RMAT graphs, scale uniform in "
+                    f"[{lo},{hi}] (2^scale vertices), edge factor uniform in "
+                    "{4,8,16,32}, a/b/c/d 0.57/0.19/0.19/0.05, seed 5; "
+                    f"queries {'+'.join(names)} per graph",
+        "queries": nq, "lanes": args.batch_lanes,
+        "hip_graph": bool(args.batch_use_graph),
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4"),
+        "qps": nq / med(t_batch),
+        "batch_s": med(t_batch),
         "per_graph_launch_qps": nq / med(t_single),
+        "per_graph_launch_s": med(t_single),
         "speedup_vs_per_graph_launch": med(t_single) / med(t_batch),
         "counts_equal": counts == single,
+        "oracle_checked_queries": checked, "oracle_equal": oracle_ok,
         "batch_kernel_ms": med(kms),
         "graph_prep_s": prep_s,
         "edges_total": sum(int(g.nnz) // 2 for g in graphs),
     }
+
+
+def bench_batch(ma, args) -> int:
+    r = run_batch(ma, args)
+    out = {
+        "metric": "batched queries/s", "value": r["qps"],
+        "unit": "queries/s", "n_gpus": 1, "steps": args.batch_steps,
+        "warmup": args.batch_warmup, "ms_per_step": r["batch_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "batch: " + r["workload"]}, **r,
+    }
     print(json.dumps(out), flush=True)
-    return 0 if counts == single else 1
+    return 0 if r["counts_equal"] and r["oracle_equal"] else 1
 
 
 def bench_dag(ma, args) -> int:
@@ -282,29 +364,64 @@ def bench_patterns(ma, args) -> int:
     return 0 if agree else 1
 
 
-def bench_serve(ma, args) -> int:
+def run_serve(ma, args, rate: float) -> dict:
+    """The served stream: Poisson arrivals at `rate` req/s over resident
+    RMAT graphs (default 64, scale uniform in [12, 20], ef 16, seed 6),
+    pattern uniform over TC, P1..P8, through the continuous-batching server
+    and through one launch per request (slots = lanes = max_batch = 1),
+    same requests; latency = completion - arrival."""
     import numpy as np
     names = [p.strip() for p in args.serve_patterns.split(",")]
     pats = [ma.PATTERN_NAMES[p] for p in names]
     lo, hi = args.serve_scales
     rng = np.random.default_rng(6)
     specs = [int(rng.integers(lo, hi + 1)) for _ in range(args.serve_graphs)]
+    t0 = time.perf_counter()
     graphs = [ma.Graph.rmat(s, 16, seed=6 * 100003 + i)
               for i, s in enumerate(specs)]
+    gen_s = time.perf_counter() - t0
+    log(f"serve: {len(graphs)} graphs generated ({gen_s:.1f} s)")
     n = args.serve_requests
-    arrivals = np.cumsum(rng.exponential(1e6 / args.serve_rate, size=n))
+    arrivals = np.cumsum(rng.exponential(1e6 / rate, size=n))
     req = [(int(rng.integers(len(graphs))), pats[int(rng.integers(len(pats)))])
            for _ in range(n)]
 
-    def run(slots, max_batch, lanes):
+    def run(slots, max_batch, lanes, label):
+        import threading
         srv = ma.Server(0, slots=slots, max_batch=max_batch, lanes=lanes)
+        t0 = time.perf_counter()
         ids = [srv.add_graph(g) for g in graphs]
-        # warm-up: one request per pattern on the smallest graph
+        add_s = time.perf_counter() - t0
+        # warm-up: every pattern once on the smallest graph
         small = int(np.argmin(specs))
         srv.replay([(ids[small], p, 0) for p in pats])
-        sched = [(ids[g], p, int(t)) for (g, p), t in zip(req, arrivals)]
+        log(f"serve {label}: {len(ids)} graphs resident ({add_s:.1f} s)")
+        # Open loop: a submitter thread follows the arrival schedule while
+        # this thread collects the results in order.
+        tickets = [None] * n
         t0 = time.perf_counter()
-        out = srv.replay(sched)
+
+        def submit_all():
+            for i, (g, p) in enumerate(req):
+                due = t0 + arrivals[i] / 1e6
+                while True:
+                    left = due - time.perf_counter()
+                    if left <= 0:
+                        break
+                    time.sleep(min(left, 1e-3))
+                tickets[i] = srv.submit(ids[g], p)
+
+        th = threading.Thread(target=submit_all)
+        th.start()
+        out = []
+        for i in range(n):
+            while tickets[i] is None:
+                time.sleep(1e-4)
+            out.append(srv.wait(tickets[i]))
+            if (i + 1) % 50 == 0 or i + 1 == n:
+                log(f"serve {label}: {i + 1}/{n} requests done "
+                    f"({time.perf_counter() - t0:.1f} s)")
+        th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
         srv.close()
@@ -315,40 +432,55 @@ def bench_serve(ma, args) -> int:
                 "p99_ms": float(np.percentile(lat, 99)),
                 "mean_batch": float(np.mean([r.batch_size for r in out])),
                 "launches": int(st.launches), "all_ok": ok,
-                "busy_ms": st.busy_ms}, [r.count for r in out]
+                "busy_ms": st.busy_ms, "deferred": int(st.deferred),
+                "add_graphs_s": add_s}, [r.count for r in out]
 
-    served, counts = run(args.serve_slots, 1024, args.serve_lanes)
-    single, counts1 = run(1, 1, 1)
-    # each result is the same alone: spot-check distinct (graph, pattern)
-    dgs = {}
-    alone_ok = True
-    for (g, p), c in list(zip(req, counts))[:200]:
+    served, counts = run(args.serve_slots, 1024, args.serve_lanes, "batched")
+    single, counts1 = run(1, 1, 1, "one launch per request")
+    # each result is the same alone: one single-query call per distinct
+    # (graph, pattern) among the first requests
+    dgs, alone_ok, seen = {}, True, set()
+    for (g, p), c in zip(req, counts):
+        if len(seen) >= args.serve_alone_checks:
+            break
+        if (g, p) in seen:
+            continue
+        seen.add((g, p))
         if g not in dgs:
             dgs[g] = ma.DeviceGraph(graphs[g], 0)
         alone_ok &= dgs[g].pattern_count(p)[0] == c
-    out = {
-        "metric": "served queries/s",
-        "value": served["qps"], "unit": "queries/s",
-        "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": served["wall_s"] * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic",
-        "config": {"workload": f"serve: {args.serve_graphs} resident RMAT "
-                               f"graphs (scale uniform in [{lo},{hi}], ef 16, "
-                               f"seed 6), Poisson arrivals at "
-                               f"{args.serve_rate:g} req/s, {n} requests, "
-                               f"pattern uniform in {{{','.join(names)}}}",
-                   "slots": args.serve_slots, "lanes": args.serve_lanes,
-                   "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4")},
-        "p50_ms": served["p50_ms"], "p99_ms": served["p99_ms"],
-        "served": served,
-        "one_launch_per_request": single,
+    for dg in dgs.values():
+        dg.close()
+    return {
+        "workload": f"{args.serve_graphs} resident RMAT graphs (scale "
+                    f"uniform in [{lo},{hi}], ef 16, seed 6), Poisson "
+                    f"arrivals at {rate:g} req/s, {n} requests, pattern "
+                    f"uniform in {{{','.join(names)}}}",
+        "rate": rate, "requests": n,
+        "slots": args.serve_slots, "lanes": args.serve_lanes,
+        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4"),
+        "qps": served["qps"], "p50_ms": served["p50_ms"],
+        "p99_ms": served["p99_ms"],
+        "served": served, "one_launch_per_request": single,
         "qps_speedup": served["qps"] / single["qps"],
         "counts_equal_modes": counts == counts1,
-        "counts_equal_alone_first200": alone_ok,
+        "alone_checked": len(seen), "counts_equal_alone": alone_ok,
+        "graph_gen_s": gen_s,
+    }
+
+
+def bench_serve(ma, args) -> int:
+    r = run_serve(ma, args, args.serve_rate)
+    out = {
+        "metric": "served queries/s", "value": r["qps"], "unit": "queries/s",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": r["served"]["wall_s"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "serve: " + r["workload"]}, **r,
     }
     print(json.dumps(out), flush=True)
-    return 0 if (counts == counts1 and alone_ok) else 1
+    return 0 if (r["counts_equal_modes"] and r["counts_equal_alone"]) else 1
 
 
 def main() -> int:
@@ -374,11 +506,12 @@ def main() -> int:
                          "the continuous-batching server vs one launch per "
                          "request")
     ap.add_argument("--serve-graphs", type=int, default=64)
-    ap.add_argument("--serve-scales", type=int, nargs=2, default=(8, 12),
+    ap.add_argument("--serve-scales", type=int, nargs=2, default=(12, 20),
                     metavar=("LO", "HI"))
     ap.add_argument("--serve-patterns", default="TC,P1,P2,P3,P4,P5,P6,P7,P8")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-alone-checks", type=int, default=60)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-lanes", type=int, default=4)
     ap.add_argument("--batch-lanes", type=int, default=8)
@@ -387,8 +520,10 @@ def main() -> int:
                          "process (0 = leave the environment's value)")
     ap.add_argument("--batch-use-graph", type=int, default=0)
     ap.add_argument("--batch-graphs", type=int, default=512)
-    ap.add_argument("--batch-scales", type=int, nargs=2, default=(10, 13),
+    ap.add_argument("--batch-scales", type=int, nargs=2, default=(10, 18),
                     metavar=("LO", "HI"))
+    ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-patterns", default="TC,P2,P4")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB "

@@ -156,8 +156,51 @@ int mgpm_graph_from_csr(int32_t n, int64_t nnz, const int64_t *rowptr,
  * dropped.  Errors: MGPM_ERR_IO, MGPM_ERR_FORMAT. */
 int mgpm_graph_read_edges(const char *path, int32_t format, int32_t base,
                           mgpm_graph **out);
-/* Writes the reference format (meta: "n 2m 4 8 1 2 maxdeg 0 0 0"). */
+/* Writes the reference format (meta: "n 2m 4 8 1 2 maxdeg 0 nvc nec", the
+ * class counts 0 when the graph has no labels of that kind) and, when the
+ * graph has labels, <prefix>.vlabel.bin / <prefix>.elabel.bin. */
 int mgpm_graph_write(const mgpm_graph *g, const char *prefix);
+
+/* ---- labels ---------------------------------------------------------------
+ * Reference: graph.cc:37-82 (use_vlabel / use_elabel in Graph::Graph).
+ * mgpm_graph_load plus labels: MGPM_LOAD_VLABELS reads <prefix>.vlabel.bin
+ * (n uint8), MGPM_LOAD_ELABELS reads <prefix>.elabel.bin (2m uint16, one
+ * per CSR entry).  As the reference asserts, it fails (MGPM_ERR_FORMAT)
+ * unless 0 < num_vertex_classes < 255 and the vertex label file holds
+ * exactly that many distinct labels, or unless 0 < num_edge_classes <
+ * 65535 and the edge label file holds at most that many.  A missing label
+ * file gets labels in 1..classes, as the reference's fallback does, drawn
+ * from a hash of the vertex / entry index (deterministic; the reference
+ * uses rand()).  Short label files fail with MGPM_ERR_IO. */
+#define MGPM_LOAD_VLABELS 1
+#define MGPM_LOAD_ELABELS 2
+int mgpm_graph_load_ex(const char *prefix, int32_t flags, mgpm_graph **out);
+/* Borrowed views of the labels (NULL, class count 0, when the graph has
+ * none of that kind), valid until mgpm_graph_free or the next
+ * mgpm_graph_set_labels.  Any output pointer may be NULL. */
+int mgpm_graph_labels(const mgpm_graph *g, const uint8_t **vlabels,
+                      int32_t *num_vertex_classes, const uint16_t **elabels,
+                      int32_t *num_edge_classes);
+/* Copies labels into g (vlabels: n entries; elabels: nnz entries; a NULL
+ * array with class count 0 removes that kind), checked by the rules that
+ * mgpm_graph_load_ex applies to label files.  Not concurrently with other
+ * calls on g. */
+int mgpm_graph_set_labels(mgpm_graph *g, int32_t num_vertex_classes,
+                          const uint8_t *vlabels, int32_t num_edge_classes,
+                          const uint16_t *elabels);
+/* Vertices per label 0..num_vertex_classes into freq (num_vertex_classes+1
+ * entries); Graph::computeLabelsFrequency (graph.cc:440-457).  Fails with
+ * MGPM_ERR_FORMAT where the reference asserts (a label above
+ * num_vertex_classes), MGPM_ERR_INVALID without vertex labels. */
+int mgpm_graph_label_frequency(const mgpm_graph *g, int64_t *freq);
+/* Neighbourhood label frequency of every vertex (Graph::BuildNLF,
+ * graph.cc:483-495) as a CSR: offsets (n+1 int64), then per vertex its
+ * distinct neighbour labels ascending (labels) with the number of
+ * neighbours carrying each (counts).  offsets is always filled; call with
+ * labels = counts = NULL first to learn offsets[n], the entries labels and
+ * counts must hold.  MGPM_ERR_INVALID without vertex labels. */
+int mgpm_graph_nlf(const mgpm_graph *g, int64_t *offsets, int32_t *labels,
+                   int32_t *counts);
 int mgpm_graph_info_get(const mgpm_graph *g, mgpm_graph_info *info);
 /* Borrowed views of the CSR, valid until mgpm_graph_free. */
 int mgpm_graph_csr(const mgpm_graph *g, const int64_t **rowptr,
@@ -269,11 +312,13 @@ int mgpm_autotune(mgpm_dgraph *dg, int pattern, int32_t reps, mgpm_tune *best,
  * No counterpart in the reference (one graph, one pattern per process).
  * Many (graph, pattern) queries run in one call with no host round trip per
  * query: one fill kernel zeroes every query's queues and counters, the
- * queries are spread over `lanes` streams (longest first, each to the
- * least-loaded lane) where their counting kernels -- the same kernels as
+ * work is spread over `lanes` streams (longest first, each to the
+ * least-loaded lane) where the counting kernels -- the same kernels as
  * mgpm_tc_count / mgpm_pattern_count, with each launch capped at the
  * workgroups its queue can feed -- overlap on the device, and one copy
- * returns all counters.  With use_graph the fill, fork, kernels and join
+ * returns all counters.  The P1..P8 queries of one graph share one census
+ * run (the union of the passes they need), each combining its own count
+ * from the census accumulators.  With use_graph the fill, fork, kernels and join
  * are recorded once into a HIP graph and replayed while the query set
  * (graph handles, patterns, plan generations) is unchanged.  Every query
  * uses its graph's resident plan (mgpm_autotune's choice if it ran).  All

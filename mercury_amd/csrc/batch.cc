@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <numeric>
+#include <unordered_map>
 
 using namespace mgpm;
 
@@ -45,17 +46,24 @@ uint64_t query_cost(const mgpm_dgraph *g, int pattern) {
 
 namespace {
 
-// Census work relative to the triangle count's: the passes each pattern
-// runs over the same DAG (triangles and sums; 4-cycles; 4- and 5-cliques;
-// the pair passes).
-uint64_t census_cost(const mgpm_dgraph *g, int pattern) {
+// Census work relative to the triangle count's: the passes a census runs
+// over the same DAG (triangles and sums; 4-cycles; 4- and 5-cliques; the
+// pair passes).
+uint64_t census_cost(const mgpm_dgraph *g, int passes) {
   const uint64_t tri = g->tc.probes + g->tc.m_plus + 1;
-  switch (pattern) {
-    case MGPM_P1: return 2 * tri;
-    case MGPM_P4: return 3 * tri;
-    case MGPM_P7: case MGPM_P8: return 8 * tri;
-    default: return 24 * tri;
-  }
+  if (passes & kCensusPassPairs) return 24 * tri;
+  return tri * (2 + ((passes & kCensusPassC4) ? 1 : 0) +
+                ((passes & kCensusPassK5) ? 6 : 0));
+}
+
+// The passes a lane's workspace is sized for per census query: P2, P3, P5
+// and P6 need every pass; the others at most the 4-cycle and 5-clique
+// passes, whatever other patterns of the same graph join them in a job.
+int need_passes(int pattern) {
+  const int all = census_passes(MGPM_P2);
+  return (census_passes(pattern) & kCensusPassPairs)
+             ? all
+             : (kCensusPassC4 | kCensusPassK5);
 }
 
 }  // namespace
@@ -106,8 +114,8 @@ void BatchExec::query_need(const mgpm_query &q, uint64_t *want_census,
   const int p = q.pattern;
   if (!g || p <= MGPM_TC || p > MGPM_P8) return;
   if (census_) {
-    *want_census = std::max(*want_census,
-                            census_work_bytes(g, p, kLaneCensusBudget));
+    *want_census = std::max(
+        *want_census, census_work_bytes(g, need_passes(p), kLaneCensusBudget));
   } else {
     const int64_t cap = std::max<int64_t>(g->host->max_degree(), 64);
     const uint64_t per_block = slab_bytes(cap, 1);
@@ -164,17 +172,24 @@ int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
   if (growth(q, nq) > max_bytes)
     return fail(MGPM_ERR_CAPACITY, "batch workspace exceeds the HBM budget");
   queries_.assign(static_cast<size_t>(nq), Planned{});
-  any_census_ = false;
+  jobs_.clear();
+  // Census queries of one graph share one census job: the union of their
+  // passes runs once and every query combines its count from it.
+  std::unordered_map<const mgpm_dgraph *, int> job_of;
   for (int i = 0; i < nq; ++i) {
     mgpm_dgraph *g = q[i].graph;
     const int p = q[i].pattern;
-    Planned pq{g, p, 0, false};
+    Planned pq{g, p, 0, -1};
     if (p == MGPM_TC) {
       pq.cost = query_cost(g, p);
     } else if (census_) {
-      pq.census = true;
-      pq.cost = census_cost(g, p);
-      any_census_ = true;
+      auto it = job_of.find(g);
+      if (it == job_of.end()) {
+        it = job_of.emplace(g, static_cast<int>(jobs_.size())).first;
+        jobs_.push_back(Job{g, 0, 0});
+      }
+      pq.job = it->second;
+      jobs_[pq.job].passes |= census_passes(p);
     } else {
       const int rc = pattern_make_resident(g, nullptr, p);
       if (rc) return rc;
@@ -182,39 +197,49 @@ int BatchExec::plan(const mgpm_query *q, int nq, uint64_t max_bytes) {
     }
     queries_[i] = pq;
   }
+  for (Job &j : jobs_) j.cost = census_cost(j.g, j.passes);
+  // Units of lane work: triangle and per-edge pattern queries (index i)
+  // and census jobs (index -1 - j).
+  struct Unit {
+    int id;
+    uint64_t cost;
+    bool heavy_pattern;
+  };
+  std::vector<Unit> units;
+  for (int i = 0; i < nq; ++i)
+    if (queries_[i].job < 0)
+      units.push_back({i, queries_[i].cost, queries_[i].pattern != MGPM_TC});
+  for (int j = 0; j < static_cast<int>(jobs_.size()); ++j)
+    units.push_back({-1 - j, jobs_[j].cost, false});
   // Pattern-heavy batches of per-edge kernels run on at most kPatternLanes
   // lanes: their kernels are throughput-bound with large working sets, and
   // more concurrent ones measured slower (TC + P2 + P4 on 512 graphs: 4
   // lanes 192 queries/s, 8 lanes 170, 16 lanes 151); triangle counts and
-  // census queries gain from every lane.
-  uint64_t pat_cost = 0, tc_cost = 0;
-  for (const Planned &pq : queries_)
-    (pq.pattern == MGPM_TC || pq.census ? tc_cost : pat_cost) += pq.cost;
-  const int use = pat_cost > tc_cost ? std::min(L, kPatternLanes) : L;
-  // Longest processing time first, each query to the least-loaded lane.
-  std::vector<int> idx(static_cast<size_t>(nq));
-  std::iota(idx.begin(), idx.end(), 0);
-  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
-    return queries_[a].cost > queries_[b].cost;
-  });
+  // census jobs gain from every lane.
+  uint64_t pat_cost = 0, other_cost = 0;
+  for (const Unit &u : units) (u.heavy_pattern ? pat_cost : other_cost) += u.cost;
+  const int use = pat_cost > other_cost ? std::min(L, kPatternLanes) : L;
+  // Longest processing time first, each unit to the least-loaded lane.
+  std::stable_sort(units.begin(), units.end(),
+                   [](const Unit &a, const Unit &b) { return a.cost > b.cost; });
   std::vector<uint64_t> load(static_cast<size_t>(use), 0);
   for (auto &o : order_) o.clear();
   std::vector<int64_t> need_cap(static_cast<size_t>(L), 0);
-  for (int i : idx) {
+  for (const Unit &u : units) {
     const int l = static_cast<int>(
         std::min_element(load.begin(), load.end()) - load.begin());
-    load[l] += queries_[i].cost;
-    order_[l].push_back(i);
-    if (queries_[i].pattern != MGPM_TC && !queries_[i].census)
-      need_cap[l] = std::max<int64_t>(need_cap[l], queries_[i].g->pat.cap);
+    load[l] += u.cost;
+    order_[l].push_back(u.id);
+    if (u.id >= 0 && u.heavy_pattern)
+      need_cap[l] = std::max<int64_t>(need_cap[l], queries_[u.id].g->pat.cap);
   }
-  // Census queries: the graph's DAG (built once, kept with the graph) and
-  // the lane's workspace, grown to the largest query on the lane.
+  // Census jobs: the graph's DAG (built once, kept with the graph) and the
+  // lane's workspace, grown to the largest job on the lane.
   for (int l = 0; l < L; ++l)
-    for (int i : order_[l])
-      if (queries_[i].census) {
-        const int rc = census_prepare(queries_[i].g, queries_[i].pattern,
-                                      census_work_[l], lanes_[l]);
+    for (int u : order_[l])
+      if (u < 0) {
+        const Job &j = jobs_[-1 - u];
+        const int rc = census_prepare(j.g, j.passes, census_work_[l], lanes_[l]);
         if (rc) return rc;
       }
   for (int l = 0; l < L; ++l) {
@@ -256,7 +281,20 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
     if (order_[l].empty()) continue;
     hipStream_t ls = lanes_[l];
     MGPM_HIP(hipStreamWaitEvent(ls, fork_, 0));
-    for (int i : order_[l]) {
+    for (int u : order_[l]) {
+      if (u < 0) {
+        // The lane's jobs run in order on its one census workspace; the
+        // accumulators are copied out before the next job reuses it.
+        const int j = -1 - u;
+        int rc = census_enqueue(jobs_[j].g, jobs_[j].passes, census_work_[l],
+                                ls);
+        if (rc) return rc;
+        MGPM_HIP(hipMemcpyAsync(accs + kCensusSlots * size_t(j),
+                                census_acc(census_work_[l]), kQueryAcc,
+                                hipMemcpyDeviceToDevice, ls));
+        continue;
+      }
+      const int i = u;
       const Planned &qp = queries_[i];
       if (qp.pattern == MGPM_TC) {
         TcDevicePlan p = qp.g->tc.dev;
@@ -270,14 +308,6 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
         tl.reset = false;
         if (tc_device_launch(p, tl, ls))
           return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
-      } else if (qp.census) {
-        // The lane's queries run in order on its one census workspace; the
-        // accumulators are copied out before the next query reuses it.
-        int rc = census_enqueue(qp.g, qp.pattern, census_work_[l], ls);
-        if (rc) return rc;
-        MGPM_HIP(hipMemcpyAsync(accs + kCensusSlots * size_t(i),
-                                census_acc(census_work_[l]), kQueryAcc,
-                                hipMemcpyDeviceToDevice, ls));
       } else {
         PatternDevicePlan p = pattern_device_plan(qp.g, qp.pattern);
         p.slab = slab_[l].as<int32_t>();
@@ -299,9 +329,9 @@ int BatchExec::enqueue(hipStream_t s, bool copy_back) {
 int BatchExec::enqueue_copy(hipStream_t s) {
   const int nq = size();
   if (nq == 0) return MGPM_OK;
-  // Counters, and the census accumulators when any query has them.
+  // Counters, and the census accumulators of the jobs.
   const size_t bytes =
-      (any_census_ ? kQueryHead + kQueryAcc : kQueryHead) * size_t(nq);
+      kQueryHead * size_t(nq) + kQueryAcc * jobs_.size();
   MGPM_HIP(hipMemcpyAsync(pinned_, scratch_.p, bytes, hipMemcpyDeviceToHost,
                           s));
   return MGPM_OK;
@@ -310,10 +340,10 @@ int BatchExec::enqueue_copy(hipStream_t s) {
 int BatchExec::result(int i, uint64_t *count) const {
   const Planned &qp = queries_[i];
   const int nq = size();
-  if (qp.census) {
+  if (qp.job >= 0) {
     const unsigned long long *acc = reinterpret_cast<const unsigned long long *>(
         reinterpret_cast<const char *>(pinned_) + kQueryHead * nq +
-        kQueryAcc * size_t(i));
+        kQueryAcc * size_t(qp.job));
     return census_combine(qp.pattern, acc, count);
   }
   const uint64_t c1 = pinned_[2 * i], c2 = pinned_[2 * i + 1];

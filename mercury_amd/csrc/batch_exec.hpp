@@ -71,10 +71,17 @@ class BatchExec {
     mgpm_dgraph *g = nullptr;
     int pattern = 0;
     uint64_t cost = 0;
-    bool census = false;
+    int job = -1;  // census job that gives its count (-1: own kernels)
+  };
+  // One census run per graph in the batch, with the union of the passes its
+  // queries need.
+  struct Job {
+    mgpm_dgraph *g = nullptr;
+    int passes = 0;
+    uint64_t cost = 0;
   };
   bool census_ = true;
-  bool any_census_ = false;          // some planned query runs the census
+  std::vector<Job> jobs_;
   std::unique_ptr<CensusWork[]> census_work_;  // per lane
   int device_ = 0;
   int cus_ = 0;
@@ -84,7 +91,8 @@ class BatchExec {
   std::unique_ptr<DevBuf[]> slab_;   // per lane
   std::vector<int64_t> slab_cap_;    // ints per list the slab is laid out for
   std::vector<int> slab_grid_;       // workgroups the slab holds
-  std::vector<std::vector<int>> order_;  // per lane, query indices in order
+  std::vector<std::vector<int>> order_;  // per lane, in order: query i, or
+                                         // census job j as -1 - j
   std::vector<Planned> queries_;
   DevBuf scratch_;                   // per query: 2 u64 counters + 4 u32
                                      // queues, then the census accumulators

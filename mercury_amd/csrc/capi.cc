@@ -207,13 +207,100 @@ const char *mgpm_last_error(void) { return last_error_cstr(); }
 // ------------------------------------------------------------ host graphs
 
 int mgpm_graph_load(const char *prefix, mgpm_graph **out) {
+  return mgpm_graph_load_ex(prefix, 0, out);
+}
+
+int mgpm_graph_load_ex(const char *prefix, int32_t flags, mgpm_graph **out) {
   if (!prefix || !out) return fail(MGPM_ERR_INVALID, "null argument");
+  if (flags & ~(MGPM_LOAD_VLABELS | MGPM_LOAD_ELABELS))
+    return fail(MGPM_ERR_INVALID, "unknown load flags");
   MGPM_GUARD_BEGIN
   auto g = std::make_shared<CsrGraph>();
   std::string err;
-  int rc = read_graph(prefix, *g, err);
+  int rc = read_graph(prefix, *g, err,
+                      ((flags & MGPM_LOAD_VLABELS) ? kLoadVertexLabels : 0) |
+                          ((flags & MGPM_LOAD_ELABELS) ? kLoadEdgeLabels : 0));
   if (rc) return fail(rc, err);
   return wrap_graph(std::move(g), out);
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_labels(const mgpm_graph *g, const uint8_t **vlabels,
+                      int32_t *num_vertex_classes, const uint16_t **elabels,
+                      int32_t *num_edge_classes) {
+  if (!g) return fail(MGPM_ERR_INVALID, "null argument");
+  const CsrGraph &c = *g->g;
+  const bool hv = !c.vlabels.empty() || (c.n == 0 && c.num_vertex_classes);
+  const bool he = !c.elabels.empty() || (c.nnz == 0 && c.num_edge_classes);
+  if (vlabels) *vlabels = c.vlabels.empty() ? nullptr : c.vlabels.data();
+  if (num_vertex_classes) *num_vertex_classes = hv ? c.num_vertex_classes : 0;
+  if (elabels) *elabels = c.elabels.empty() ? nullptr : c.elabels.data();
+  if (num_edge_classes) *num_edge_classes = he ? c.num_edge_classes : 0;
+  return MGPM_OK;
+}
+
+int mgpm_graph_set_labels(mgpm_graph *g, int32_t num_vertex_classes,
+                          const uint8_t *vlabels, int32_t num_edge_classes,
+                          const uint16_t *elabels) {
+  if (!g) return fail(MGPM_ERR_INVALID, "null argument");
+  if ((!vlabels && num_vertex_classes) || (!elabels && num_edge_classes))
+    return fail(MGPM_ERR_INVALID, "class count without labels");
+  MGPM_GUARD_BEGIN
+  CsrGraph &c = *g->g;
+  CsrGraph t;  // checked before g changes
+  t.n = c.n;
+  t.nnz = c.nnz;
+  t.num_vertex_classes = num_vertex_classes;
+  t.num_edge_classes = num_edge_classes;
+  if (vlabels) {
+    if (!(num_vertex_classes > 0 && num_vertex_classes < 255))
+      return fail(MGPM_ERR_FORMAT,
+                  "vertex labels need 0 < num_vertex_classes < 255");
+    t.vlabels.assign(vlabels, vlabels + c.n);
+  }
+  if (elabels) {
+    if (!(num_edge_classes > 0 && num_edge_classes < 65535))
+      return fail(MGPM_ERR_FORMAT,
+                  "edge labels need 0 < num_edge_classes < 65535");
+    t.elabels.assign(elabels, elabels + c.nnz);
+  }
+  std::string err;
+  int rc = check_labels(t, err);
+  if (rc) return fail(rc, err);
+  c.num_vertex_classes = t.num_vertex_classes;
+  c.num_edge_classes = t.num_edge_classes;
+  c.vlabels.swap(t.vlabels);
+  c.elabels.swap(t.elabels);
+  return MGPM_OK;
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_label_frequency(const mgpm_graph *g, int64_t *freq) {
+  if (!g || !freq) return fail(MGPM_ERR_INVALID, "null argument");
+  const CsrGraph &c = *g->g;
+  if (c.num_vertex_classes <= 0)
+    return fail(MGPM_ERR_INVALID, "graph has no vertex labels");
+  MGPM_GUARD_BEGIN
+  std::vector<int64_t> f;
+  if (!label_frequency(c, f))
+    return fail(MGPM_ERR_FORMAT, "vertex label above num_vertex_classes");
+  std::copy(f.begin(), f.end(), freq);
+  return MGPM_OK;
+  MGPM_GUARD_END
+}
+
+int mgpm_graph_nlf(const mgpm_graph *g, int64_t *offsets, int32_t *labels,
+                   int32_t *counts) {
+  if (!g || !offsets) return fail(MGPM_ERR_INVALID, "null argument");
+  const CsrGraph &c = *g->g;
+  if (c.num_vertex_classes <= 0)
+    return fail(MGPM_ERR_INVALID, "graph has no vertex labels");
+  if (!labels != !counts)
+    return fail(MGPM_ERR_INVALID, "labels and counts go together");
+  MGPM_GUARD_BEGIN
+  nlf_csr(c, offsets, nullptr, nullptr);
+  if (labels) nlf_csr(c, offsets, labels, counts);
+  return MGPM_OK;
   MGPM_GUARD_END
 }
 
@@ -404,8 +491,7 @@ int mgpm_dgraph_upload_ex(const mgpm_graph *g, int device, void *stream,
 int mgpm_dgraph_footprint(const mgpm_dgraph *dg, uint64_t *bytes) {
   if (!dg || !bytes) return fail(MGPM_ERR_INVALID, "null argument");
   *bytes = dg->tc.bytes() + dg->csr.bytes() + dg->pat.bytes() +
-           dg->dag.bytes() + dg->census.bytes() + dg->census_work.bytes() +
-           dg->scratch.bytes;
+           dg->dag.bytes() + dg->census.bytes() + dg->scratch.bytes;
   return MGPM_OK;
 }
 

@@ -152,13 +152,13 @@ struct CensusGraph {
   }
 };
 
-// Census workspace of one caller (a graph's single-query path, a batch
-// lane, a server slot): grows to the largest query it has served and is
-// reused without allocating.
+// Census workspace of one caller (the single-query path of a device, a
+// batch lane, a server slot): grows to the largest query it has served and
+// is reused without allocating.
 struct CensusWork {
   // Bytes the pair passes' per-workgroup arrays may take (more workgroups
-  // in flight on larger budgets): a graph's own single-query workspace
-  // takes 16 GiB, batch lanes and server slots less.
+  // in flight on larger budgets): the single-query workspace takes 16 GiB,
+  // batch lanes and server slots less.
   size_t budget = size_t(16) << 30;
   DevBuf t, t_csr, tv, small, slab, wedges, touched, k5_slab;
   DevBuf stamp, lo, mask, list, grows, anchors, table;
@@ -208,25 +208,33 @@ inline int default_tc_items(uint64_t probes, uint64_t partners) {
 // passes for P2, P3, P5 and P6.
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms);
-// The census split for callers that run many queries: census_prepare
-// builds what the graph keeps (DAG, pair-table layout; may allocate and
-// synchronise `s`) and grows `w` for the pattern; census_enqueue then
-// enqueues the whole count on `s` with no allocation and no host
-// synchronisation, leaving the accumulators at census_acc(w);
-// census_combine turns them into the count (MGPM_ERR_INTERNAL if the
-// census's self-checks disagree).
-int census_prepare(mgpm_dgraph *dg, int pattern, CensusWork &w, hipStream_t s);
+// The census split for callers that run many queries.  A census runs the
+// triangle pass and its sums plus the passes of a census_passes mask; one
+// run serves every pattern whose passes it includes (census_passes of P2,
+// P3, P5 and P6 includes every pass, so one run serves all eight).
+// census_prepare builds what the graph keeps (DAG, pair-table layout; may
+// allocate and synchronise `s`) and grows `w` for the passes;
+// census_enqueue then enqueues the whole census on `s` with no allocation
+// and no host synchronisation, leaving the accumulators at census_acc(w);
+// census_combine turns them into a pattern's count (MGPM_ERR_INTERNAL if
+// the census's self-checks disagree).
+enum CensusPass : int {
+  kCensusPassC4 = 1,     // 4-cycles (P4)
+  kCensusPassK5 = 2,     // 4-cliques per triangle, 5-cliques (P7, P8)
+  kCensusPassPairs = 4,  // the 5-vertex pair passes (P2, P3, P5, P6)
+};
+int census_passes(int pattern);
+int census_prepare(mgpm_dgraph *dg, int passes, CensusWork &w, hipStream_t s);
 // What census_prepare builds for the graph alone (DAG and pair-table
 // layout), with temporary buffers only.
 int census_graph_prepare(mgpm_dgraph *dg, hipStream_t s);
-int census_enqueue(mgpm_dgraph *dg, int pattern, CensusWork &w, hipStream_t s);
+int census_enqueue(mgpm_dgraph *dg, int passes, CensusWork &w, hipStream_t s);
 unsigned long long *census_acc(const CensusWork &w);
 int census_combine(int pattern, const unsigned long long *acc,
                    uint64_t *count);
-// Device bytes of a census workspace for this pattern on this graph
+// Device bytes of a census workspace for these passes on this graph
 // (before the graph's census DAG exists: a bound that includes it).
-uint64_t census_work_bytes(const mgpm_dgraph *dg, int pattern,
-                           size_t budget);
+uint64_t census_work_bytes(const mgpm_dgraph *dg, int passes, size_t budget);
 // Rebuilds the resident triangle plan where it was built (host or device).
 int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s);
 // Builds pattern `pattern`'s plan if missing or if `split` (0 = the
@@ -253,7 +261,6 @@ struct mgpm_dgraph {
   mgpm::PatternResident pat;
   mgpm::DagResident dag;
   mgpm::CensusGraph census;            // DAG of the census
-  mgpm::CensusWork census_work;        // single-query census workspace
   mgpm::DevBuf scratch;               // queues + result counters
   unsigned long long *pinned = nullptr;  // host mirror of results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -40,6 +40,7 @@
 // inverse modulo 2^64 (exact for a multiple of 3 whatever the wrap).
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "capi_common.hpp"
 #include "census.hpp"
@@ -48,6 +49,8 @@
 namespace mgpm {
 
 namespace {
+
+constexpr int kMaxCensusDevices = 64;
 
 // Per-caller budgets of the per-workgroup arrays.
 constexpr size_t kCensusC4Budget = size_t(4) << 30;
@@ -62,11 +65,12 @@ constexpr size_t kC5CursorOff = 384;
 constexpr size_t kClsCountOff = 448;
 constexpr size_t kClsFillOff = 480;
 
-bool is_k5(int p) { return p == MGPM_P7 || p == MGPM_P8; }
-bool is_full5(int p) {
-  return p == MGPM_P2 || p == MGPM_P3 || p == MGPM_P5 || p == MGPM_P6;
-}
-bool needs_c4(int p) { return p == MGPM_P4 || is_full5(p); }
+// `passes` is a census_passes mask: the triangle pass and the sums always
+// run, the 4-cycle, 5-clique and pair passes on request; the pair passes
+// also add the 5-vertex terms to the others.
+bool is_k5(int passes) { return (passes & kCensusPassK5) != 0; }
+bool is_full5(int passes) { return (passes & kCensusPassPairs) != 0; }
+bool needs_c4(int passes) { return (passes & kCensusPassC4) != 0; }
 
 // The inverse of 3 modulo 2^64.
 constexpr uint64_t kInv3 = 0xAAAAAAAAAAAAAAABull;
@@ -94,7 +98,7 @@ struct Need {
   int64_t k5_len = 0, mask_words = 0, grows_len = 0;
 };
 
-Need need_for(const mgpm_dgraph *dg, int pattern, size_t budget) {
+Need need_for(const mgpm_dgraph *dg, int passes, size_t budget) {
   const CsrGraph &G = *dg->host;
   const int64_t n = std::max<int32_t>(G.n, 1);
   const int64_t m = std::max<int64_t>(G.nnz / 2, 1);
@@ -103,13 +107,13 @@ Need need_for(const mgpm_dgraph *dg, int pattern, size_t budget) {
   Need k;
   k.t = static_cast<size_t>(m);
   k.slab = static_cast<size_t>(census_waves(dg->cus)) * cg.max_out;
-  if (needs_c4(pattern)) {
+  if (needs_c4(passes)) {
     const size_t per = static_cast<size_t>(n) * sizeof(uint32_t);
     k.c4_groups = static_cast<int>(std::min<size_t>(
         1024, std::max<size_t>(64, kCensusC4Budget / (2 * per))));
     k.wedges = static_cast<size_t>(k.c4_groups) * n;
   }
-  if (is_k5(pattern) || is_full5(pattern)) {
+  if (is_k5(passes)) {
     k.k5_len = census_k5_slab_len(cg.max_out, cg.max_deg);
     const size_t per = static_cast<size_t>(k.k5_len) * sizeof(int32_t);
     const size_t fit = kCensusK5Budget / per;
@@ -118,7 +122,7 @@ Need need_for(const mgpm_dgraph *dg, int pattern, size_t budget) {
         ~size_t(3));
     k.k5 = static_cast<size_t>(k.k5_waves) * k.k5_len;
   }
-  if (is_full5(pattern)) {
+  if (is_full5(passes)) {
     k.t_csr = static_cast<size_t>(nnz);
     k.tv = static_cast<size_t>(n);
     const int gen_words = static_cast<int>((cg.max_out + 63) / 64);
@@ -203,7 +207,7 @@ int graph_ready(mgpm_dgraph *dg, hipStream_t s) {
 
 unsigned *cursor_of(const CensusWork &w) { return w.small.as<unsigned>(); }
 
-CensusArgs args_of(mgpm_dgraph *dg, int pattern, CensusWork &w,
+CensusArgs args_of(mgpm_dgraph *dg, int passes, CensusWork &w,
                    const Need &k) {
   const CensusGraph &cg = dg->census;
   CensusArgs c;
@@ -227,7 +231,7 @@ CensusArgs args_of(mgpm_dgraph *dg, int pattern, CensusWork &w,
   c.k5_len = k.k5_len;
   c.max_deg = cg.max_deg;
   c.k5_waves = k.k5_waves;
-  c.full5 = is_full5(pattern);
+  c.full5 = is_full5(passes);
   c.t_csr = w.t_csr.as<uint32_t>();
   c.tv = w.tv.as<unsigned long long>();
   return c;
@@ -306,18 +310,18 @@ unsigned long long *census_acc(const CensusWork &w) {
   return reinterpret_cast<unsigned long long *>(w.small.as<char>() + kAccOff);
 }
 
-uint64_t census_work_bytes(const mgpm_dgraph *dg, int pattern,
+uint64_t census_work_bytes(const mgpm_dgraph *dg, int passes,
                            size_t budget) {
   if (!dg->census.ready) {
     // The DAG the graph will keep, plus a bound on the workspace.
     const CsrGraph &G = *dg->host;
     const uint64_t dag = 4ull * G.n + 4ull * G.nnz + 8ull * (G.n + 1) +
                          8ull * (G.nnz / 2);
-    const uint64_t work = is_full5(pattern) ? budget + 16ull * G.nnz
-                                            : 32ull * G.nnz;
+    const uint64_t work = is_full5(passes) ? budget + 16ull * G.nnz
+                                           : 32ull * G.nnz;
     return dag + work;
   }
-  const Need k = need_for(dg, pattern, budget);
+  const Need k = need_for(dg, passes, budget);
   return 4ull * k.t + kSmallBytes + 4ull * k.slab + 8ull * k.wedges +
          4ull * k.k5 + 4ull * k.t_csr + 8ull * k.tv + 12ull * k.stamp +
          8ull * k.mask + 8ull * k.grows + 4ull * k.anchors + 8ull * k.table;
@@ -385,24 +389,32 @@ int census_graph_prepare(mgpm_dgraph *dg, hipStream_t s) {
   return pairs_layout(dg, s);
 }
 
-int census_prepare(mgpm_dgraph *dg, int pattern, CensusWork &w,
-                   hipStream_t s) {
-  if (pattern < MGPM_P1 || pattern > MGPM_P8)
-    return fail(MGPM_ERR_PATTERN, "Not support.");
-  int rc = graph_ready(dg, s);
-  if (rc) return rc;
-  if (is_full5(pattern) && (rc = pairs_layout(dg, s))) return rc;
-  return work_grow(w, need_for(dg, pattern, w.budget), s);
+int census_passes(int pattern) {
+  switch (pattern) {
+    case MGPM_P1: return 0;
+    case MGPM_P4: return kCensusPassC4;
+    case MGPM_P7: case MGPM_P8: return kCensusPassK5;
+    default:  // P2, P3, P5, P6
+      return kCensusPassC4 | kCensusPassK5 | kCensusPassPairs;
+  }
 }
 
-int census_enqueue(mgpm_dgraph *dg, int pattern, CensusWork &w,
+int census_prepare(mgpm_dgraph *dg, int passes, CensusWork &w,
                    hipStream_t s) {
-  const Need k = need_for(dg, pattern, w.budget);
-  CensusArgs c = args_of(dg, pattern, w, k);
+  int rc = graph_ready(dg, s);
+  if (rc) return rc;
+  if (is_full5(passes) && (rc = pairs_layout(dg, s))) return rc;
+  return work_grow(w, need_for(dg, passes, w.budget), s);
+}
+
+int census_enqueue(mgpm_dgraph *dg, int passes, CensusWork &w,
+                   hipStream_t s) {
+  const Need k = need_for(dg, passes, w.budget);
+  CensusArgs c = args_of(dg, passes, w, k);
   int rc;
   if ((rc = zero(w.t.p, static_cast<size_t>(c.m) * 4, s))) return rc;
   if ((rc = zero(w.small.p, kSmallBytes, s))) return rc;
-  const bool f5 = is_full5(pattern);
+  const bool f5 = is_full5(passes);
   if (f5 && (rc = zero(w.tv.p, static_cast<size_t>(c.n) * 8, s))) return rc;
   if (census_launch(c, dg->cus, s))
     return fail(MGPM_ERR_HIP, "census kernel launch failed");
@@ -412,9 +424,9 @@ int census_enqueue(mgpm_dgraph *dg, int pattern, CensusWork &w,
     if (census5_flat_launch(c5, dg->cus, s))
       return fail(MGPM_ERR_HIP, "census kernel launch failed");
   }
-  if (needs_c4(pattern) && census_c4_launch(c, s))
+  if (needs_c4(passes) && census_c4_launch(c, s))
     return fail(MGPM_ERR_HIP, "census kernel launch failed");
-  if ((is_k5(pattern) || f5) && census_k5_launch(c, s))
+  if (is_k5(passes) && census_k5_launch(c, s))
     return fail(MGPM_ERR_HIP, "census kernel launch failed");
   if (f5) {
     const CensusGraph &cg = dg->census;
@@ -502,13 +514,38 @@ int census_combine(int pattern, const unsigned long long *acc,
   return MGPM_OK;
 }
 
+namespace {
+
+// The single-query path's workspace: one per device, shared by every
+// graph on it (a workspace per graph would hold gigabytes per handle), so
+// single-query census calls on one device run one at a time.  Never freed:
+// it lives until the process ends.
+struct SharedWork {
+  std::mutex mu;
+  CensusWork w;
+};
+
+SharedWork &shared_work(int device) {
+  static SharedWork *all = new SharedWork[kMaxCensusDevices];
+  return all[device];
+}
+
+}  // namespace
+
 int census_pattern_count(mgpm_dgraph *dg, int pattern, hipStream_t s,
                          uint64_t *count, float *kernel_ms) {
-  CensusWork &w = dg->census_work;
-  int rc = census_prepare(dg, pattern, w, s);
+  if (pattern < MGPM_P1 || pattern > MGPM_P8)
+    return fail(MGPM_ERR_PATTERN, "Not support.");
+  if (dg->device < 0 || dg->device >= kMaxCensusDevices)
+    return fail(MGPM_ERR_NODEVICE, "census: device index out of range");
+  SharedWork &sw = shared_work(dg->device);
+  std::lock_guard<std::mutex> lk(sw.mu);
+  CensusWork &w = sw.w;
+  const int passes = census_passes(pattern);
+  int rc = census_prepare(dg, passes, w, s);
   if (rc) return rc;
   MGPM_HIP(hipEventRecord(dg->ev0, s));
-  if ((rc = census_enqueue(dg, pattern, w, s))) return rc;
+  if ((rc = census_enqueue(dg, passes, w, s))) return rc;
   MGPM_HIP(hipEventRecord(dg->ev1, s));
   unsigned long long acc[kCensusSlots];
   MGPM_HIP(hipMemcpyAsync(acc, census_acc(w), sizeof(acc),

@@ -153,11 +153,20 @@ __device__ __forceinline__ int class_of(int dp) {
   return k;
 }
 
+// Class sizes and the scatter of the anchors into their classes, with one
+// atomic per class per wave (ballots over the wave's 64 vertices).  The
+// loops run whole waves to the end (inactive lanes hold class -1).
 __global__ void c5_class_count(Census5Args c) {
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < c.n;
-       v += int64_t(gridDim.x) * blockDim.x) {
-    const int dp = static_cast<int>(c.drp[v + 1] - c.drp[v]);
-    atomicAdd(&c.cls_count[class_of(dp)], 1u);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v0 = int64_t(blockIdx.x) * blockDim.x; v0 < c.n; v0 += stride) {
+    const int64_t v = v0 + threadIdx.x;
+    const int k = v < c.n ? class_of(static_cast<int>(c.drp[v + 1] - c.drp[v]))
+                          : -1;
+    for (int q = 0; q < kC5Classes; ++q) {
+      const unsigned long long m = __ballot(k == q);
+      if (lane64() == 0 && m)
+        atomicAdd(&c.cls_count[q], static_cast<unsigned>(__popcll(m)));
+    }
   }
 }
 
@@ -168,12 +177,21 @@ __global__ void c5_class_scatter(Census5Args c) {
     start[k] = s;
     s += c.cls_count[k];
   }
-  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < c.n;
-       v += int64_t(gridDim.x) * blockDim.x) {
-    const int dp = static_cast<int>(c.drp[v + 1] - c.drp[v]);
-    const int k = class_of(dp);
-    c.anchors[start[k] + atomicAdd(&c.cls_fill[k], 1u)] =
-        static_cast<int32_t>(v);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v0 = int64_t(blockIdx.x) * blockDim.x; v0 < c.n; v0 += stride) {
+    const int64_t v = v0 + threadIdx.x;
+    const int k = v < c.n ? class_of(static_cast<int>(c.drp[v + 1] - c.drp[v]))
+                          : -1;
+    for (int q = 0; q < kC5Classes; ++q) {
+      const unsigned long long m = __ballot(k == q);
+      if (!m) continue;
+      unsigned base = 0;
+      if (lane64() == 0)
+        base = atomicAdd(&c.cls_fill[q], static_cast<unsigned>(__popcll(m)));
+      base = __shfl(base, 0, 64);
+      if (k == q)
+        c.anchors[start[q] + base + bits_below(m)] = static_cast<int32_t>(v);
+    }
   }
 }
 
@@ -302,10 +320,15 @@ __global__ __launch_bounds__(kT) void c5_anchor(Census5Args c, int cls) {
       for (int64_t k = c.rowptr[y] + lane; k < ey; k += 64) {
         const int32_t cc = c.col[k];
         if (!above(c.deg, a, cc)) continue;
-        atomicOr(&mask[size_t(cc) * rs + wd], bit);
-        if (atomicExch(&stamp[cc], tag) != tag) list[atomicAdd(&s_nt, 1u)] = cc;
+        const unsigned long long old =
+            atomicOr(&mask[size_t(cc) * rs + wd], bit);
+        // One-word masks: the first bit set on c (its mask and count were
+        // zero) lists it; wider masks stamp c with the anchor.
+        if (W == 1 ? old == 0ull : atomicExch(&stamp[cc], tag) != tag)
+          list[atomicAdd(&s_nt, 1u)] = cc;
       }
     }
+    if (W == 1) __syncthreads();  // every mask final before the counts
     // lo: common neighbours below a, through every z below a.
     const int64_t ea = c.rowptr[a + 1];
     for (int64_t k = c.rowptr[a] + wv; k < ea; k += kTW) {
@@ -316,8 +339,15 @@ __global__ __launch_bounds__(kT) void c5_anchor(Census5Args c, int cls) {
       for (int q = lane; q < dz; q += 64) {
         const int32_t cc = c.dcol[bz + q];
         if (!above(c.deg, a, cc)) continue;
-        atomicAdd(&lo[cc], 1u);
-        if (atomicExch(&stamp[cc], tag) != tag) list[atomicAdd(&s_nt, 1u)] = cc;
+        const uint32_t old = atomicAdd(&lo[cc], 1u);
+        bool first;
+        if (W == 1)  // listed already if its mask has a bit
+          first = old == 0u &&
+                  __hip_atomic_load(&mask[size_t(cc)], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == 0ull;
+        else
+          first = atomicExch(&stamp[cc], tag) != tag;
+        if (first) list[atomicAdd(&s_nt, 1u)] = cc;
       }
     }
     __syncthreads();

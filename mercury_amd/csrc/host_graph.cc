@@ -109,7 +109,137 @@ int write_exact(const std::string &path, const void *src, uint64_t bytes,
 
 }  // namespace
 
-int read_graph(const std::string &prefix, CsrGraph &g, std::string &err) {
+namespace {
+
+bool file_exists(const std::string &path) {
+  FILE *f = std::fopen(path.c_str(), "rb");
+  if (!f) return false;
+  std::fclose(f);
+  return true;
+}
+
+// Label i of `classes` for a missing label file: 1..classes.
+uint32_t fallback_label(uint64_t i, uint32_t classes, uint64_t salt) {
+  uint64_t z = i + salt + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return static_cast<uint32_t>(z % classes) + 1u;
+}
+
+}  // namespace
+
+int check_labels(const CsrGraph &g, std::string &err, bool check_vertex,
+                 bool check_edge) {
+  if (check_vertex && !g.vlabels.empty()) {
+    if (!(g.num_vertex_classes > 0 && g.num_vertex_classes < 255)) {
+      err = "vertex labels need 0 < num_vertex_classes < 255";
+      return MGPM_ERR_FORMAT;
+    }
+    if (static_cast<int64_t>(g.vlabels.size()) != g.n) {
+      err = "vertex labels: one per vertex";
+      return MGPM_ERR_FORMAT;
+    }
+    bool seen[256] = {};
+    int distinct = 0;
+    for (uint8_t l : g.vlabels)
+      if (!seen[l]) {
+        seen[l] = true;
+        ++distinct;
+      }
+    if (distinct != g.num_vertex_classes) {
+      err = "vertex labels: " + std::to_string(distinct) +
+            " distinct labels, num_vertex_classes says " +
+            std::to_string(g.num_vertex_classes);
+      return MGPM_ERR_FORMAT;
+    }
+  }
+  if (check_edge && !g.elabels.empty()) {
+    if (!(g.num_edge_classes > 0 && g.num_edge_classes < 65535)) {
+      err = "edge labels need 0 < num_edge_classes < 65535";
+      return MGPM_ERR_FORMAT;
+    }
+    if (static_cast<int64_t>(g.elabels.size()) != g.nnz) {
+      err = "edge labels: one per CSR entry";
+      return MGPM_ERR_FORMAT;
+    }
+    std::vector<uint8_t> seen(65536, 0);
+    int distinct = 0;
+    for (uint16_t l : g.elabels)
+      if (!seen[l]) {
+        seen[l] = 1;
+        ++distinct;
+      }
+    if (distinct > g.num_edge_classes) {
+      err = "edge labels: " + std::to_string(distinct) +
+            " distinct labels, more than num_edge_classes " +
+            std::to_string(g.num_edge_classes);
+      return MGPM_ERR_FORMAT;
+    }
+  }
+  return MGPM_OK;
+}
+
+std::vector<std::pair<int32_t, int32_t>> nlf(const CsrGraph &g, int32_t v) {
+  int32_t cnt[256] = {};
+  for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e)
+    ++cnt[g.vlabels[g.colidx[e]]];
+  std::vector<std::pair<int32_t, int32_t>> out;
+  for (int32_t l = 0; l < 256; ++l)
+    if (cnt[l]) out.emplace_back(l, cnt[l]);
+  return out;
+}
+
+void nlf_csr(const CsrGraph &g, int64_t *offsets, int32_t *labels,
+             int32_t *counts) {
+  const int32_t n = g.n;
+  if (!labels || !counts) {
+    offsets[0] = 0;
+#pragma omp parallel
+    {
+      uint32_t stamp[256] = {};
+#pragma omp for schedule(dynamic, 4096)
+      for (int32_t v = 0; v < n; ++v) {
+        const uint32_t tag = static_cast<uint32_t>(v) + 1;
+        int64_t k = 0;
+        for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e) {
+          const uint8_t l = g.vlabels[g.colidx[e]];
+          if (stamp[l] != tag) {
+            stamp[l] = tag;
+            ++k;
+          }
+        }
+        offsets[v + 1] = k;
+      }
+    }
+    for (int32_t v = 0; v < n; ++v) offsets[v + 1] += offsets[v];
+    return;
+  }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int32_t v = 0; v < n; ++v) {
+    int32_t cnt[256] = {};
+    for (int64_t e = g.rowptr[v]; e < g.rowptr[v + 1]; ++e)
+      ++cnt[g.vlabels[g.colidx[e]]];
+    int64_t o = offsets[v];
+    for (int32_t l = 0; l < 256; ++l)
+      if (cnt[l]) {
+        labels[o] = l;
+        counts[o++] = cnt[l];
+      }
+  }
+}
+
+bool label_frequency(const CsrGraph &g, std::vector<int64_t> &freq) {
+  freq.assign(static_cast<size_t>(g.num_vertex_classes) + 1, 0);
+  for (uint8_t l : g.vlabels) {
+    if (l >= freq.size()) return false;
+    ++freq[l];
+  }
+  return true;
+}
+
+int read_graph(const std::string &prefix, CsrGraph &g, std::string &err,
+               int flags) {
   std::ifstream meta(prefix + ".meta.txt");
   if (!meta) {
     err = "Failed to open file: " + prefix + ".meta.txt";
@@ -139,6 +269,7 @@ int read_graph(const std::string &prefix, CsrGraph &g, std::string &err) {
   g.n = static_cast<int32_t>(n);
   g.nnz = nnz;
   g.meta_max_degree = static_cast<int32_t>(md);
+  const long long nvc = f[8], nec = f[9];
   try {
     g.rowptr.assign(static_cast<size_t>(n) + 1, 0);
     g.colidx.assign(static_cast<size_t>(nnz), 0);
@@ -152,7 +283,49 @@ int read_graph(const std::string &prefix, CsrGraph &g, std::string &err) {
   rc = read_exact(prefix + ".edge.bin", g.colidx.data(),
                   static_cast<uint64_t>(nnz) * 4, err);
   if (rc) return rc;
-  return check_simple(g, err);
+  if ((rc = check_simple(g, err))) return rc;
+  bool vfile = false, efile = false;
+  if (flags & kLoadVertexLabels) {
+    if (!(nvc > 0 && nvc < 255)) {  // graph.cc:40-41
+      err = "graph.meta.txt: vertex labels need 0 < num_vertex_classes < 255";
+      return MGPM_ERR_FORMAT;
+    }
+    g.num_vertex_classes = static_cast<int32_t>(nvc);
+    g.vlabels.assign(static_cast<size_t>(n), 0);
+    const std::string path = prefix + ".vlabel.bin";
+    if (file_exists(path)) {
+      if ((rc = read_exact(path, g.vlabels.data(), static_cast<uint64_t>(n),
+                           err)))
+        return rc;
+      vfile = true;
+    } else {
+      for (int64_t v = 0; v < n; ++v)
+        g.vlabels[v] = static_cast<uint8_t>(
+            fallback_label(static_cast<uint64_t>(v), static_cast<uint32_t>(nvc), 1));
+    }
+  }
+  if (flags & kLoadEdgeLabels) {
+    if (!(nec > 0 && nec < 65535)) {  // graph.cc:62-63
+      err = "graph.meta.txt: edge labels need 0 < num_edge_classes < 65535";
+      return MGPM_ERR_FORMAT;
+    }
+    g.num_edge_classes = static_cast<int32_t>(nec);
+    g.elabels.assign(static_cast<size_t>(nnz), 0);
+    const std::string path = prefix + ".elabel.bin";
+    if (file_exists(path)) {
+      if ((rc = read_exact(path, g.elabels.data(),
+                           static_cast<uint64_t>(nnz) * 2, err)))
+        return rc;
+      efile = true;
+    } else {
+      for (int64_t e = 0; e < nnz; ++e)
+        g.elabels[e] = static_cast<uint16_t>(
+            fallback_label(static_cast<uint64_t>(e), static_cast<uint32_t>(nec), 2));
+    }
+  }
+  // generated labels are not checked against the class counts (graph.cc
+  // checks only labels read from a file)
+  return check_labels(g, err, vfile, efile);
 }
 
 int write_graph(const std::string &prefix, const CsrGraph &g,
@@ -163,8 +336,9 @@ int write_graph(const std::string &prefix, const CsrGraph &g,
       err = "cannot create file: " + prefix + ".meta.txt";
       return MGPM_ERR_IO;
     }
-    meta << g.n << " " << g.nnz << " 4 8 1 2 " << g.max_degree()
-         << " 0 0 0\n";
+    meta << g.n << " " << g.nnz << " 4 8 1 2 " << g.max_degree() << " 0 "
+         << (g.vlabels.empty() ? 0 : g.num_vertex_classes) << " "
+         << (g.elabels.empty() ? 0 : g.num_edge_classes) << "\n";
     if (!meta) {
       err = "short write: " + prefix + ".meta.txt";
       return MGPM_ERR_IO;
@@ -173,8 +347,18 @@ int write_graph(const std::string &prefix, const CsrGraph &g,
   int rc = write_exact(prefix + ".vertex.bin", g.rowptr.data(),
                        (static_cast<uint64_t>(g.n) + 1) * 8, err);
   if (rc) return rc;
-  return write_exact(prefix + ".edge.bin", g.colidx.data(),
-                     static_cast<uint64_t>(g.nnz) * 4, err);
+  if ((rc = write_exact(prefix + ".edge.bin", g.colidx.data(),
+                        static_cast<uint64_t>(g.nnz) * 4, err)))
+    return rc;
+  if (!g.vlabels.empty() &&
+      (rc = write_exact(prefix + ".vlabel.bin", g.vlabels.data(),
+                        static_cast<uint64_t>(g.n), err)))
+    return rc;
+  if (!g.elabels.empty() &&
+      (rc = write_exact(prefix + ".elabel.bin", g.elabels.data(),
+                        static_cast<uint64_t>(g.nnz) * 2, err)))
+    return rc;
+  return MGPM_OK;
 }
 
 int check_simple(const CsrGraph &g, std::string &err) {

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace mgpm {
@@ -16,6 +17,11 @@ struct CsrGraph {
   int32_t meta_max_degree = 0;  // value stored in graph.meta.txt, if loaded
   std::vector<int64_t> rowptr;  // n + 1
   std::vector<int32_t> colidx;  // nnz
+  // Optional labels (graph.cc:37-82): one 8-bit label per vertex, one
+  // 16-bit label per CSR entry; empty when not loaded or set.
+  int32_t num_vertex_classes = 0, num_edge_classes = 0;
+  std::vector<uint8_t> vlabels;   // n
+  std::vector<uint16_t> elabels;  // nnz
   int32_t degree(int32_t v) const {
     return static_cast<int32_t>(rowptr[v + 1] - rowptr[v]);
   }
@@ -27,8 +33,35 @@ struct CsrGraph {
 // num_vertex_classes num_edge_classes"; <prefix>.vertex.bin holds n+1 int64
 // row offsets and <prefix>.edge.bin holds n_edges int32 column ids.
 // Returns an MGPM status code and fills `err` on failure.
-int read_graph(const std::string &prefix, CsrGraph &g, std::string &err);
+// flags: kLoadVertexLabels / kLoadEdgeLabels also read
+// <prefix>.vlabel.bin (n uint8) / <prefix>.elabel.bin (n_edges uint16),
+// checked as graph.cc:37-82 asserts: 0 < num_vertex_classes < 255 with
+// exactly that many distinct vertex labels, 0 < num_edge_classes < 65535
+// with at most that many distinct edge labels.  A missing label file gets
+// labels 1..classes, as the reference's fallback does (graph.cc:49-55,
+// 71-77), from a hash of the index instead of rand().
+constexpr int kLoadVertexLabels = 1;
+constexpr int kLoadEdgeLabels = 2;
+int read_graph(const std::string &prefix, CsrGraph &g, std::string &err,
+               int flags = 0);
+// Writes the labels (and their class counts in the meta file) when set.
 int write_graph(const std::string &prefix, const CsrGraph &g, std::string &err);
+// Checks labels against their class counts (the rules of read_graph for
+// labels read from a file).
+int check_labels(const CsrGraph &g, std::string &err, bool check_vertex = true,
+                 bool check_edge = true);
+// Neighbourhood label frequency of v (Graph::BuildNLF, graph.cc:483-495):
+// (label, neighbours of v with that label), ascending by label.
+std::vector<std::pair<int32_t, int32_t>> nlf(const CsrGraph &g, int32_t v);
+// Every vertex's NLF as a CSR: offsets (n + 1) and, per vertex, its
+// distinct neighbour labels ascending with their counts.  Arrays sized by
+// a first call with offsets only (labels/counts null) or by offsets[n].
+void nlf_csr(const CsrGraph &g, int64_t *offsets, int32_t *labels,
+             int32_t *counts);
+// Vertices per vertex label, labels 0..num_vertex_classes
+// (Graph::computeLabelsFrequency, graph.cc:440-457); false, where the
+// reference asserts, if a label exceeds num_vertex_classes.
+bool label_frequency(const CsrGraph &g, std::vector<int64_t> &freq);
 
 // Edge files (in place of the reference's closed datasets/converter and
 // datasets/tsv2mtx.py): format 1 = MatrixMarket coordinate ("%%MatrixMarket"

@@ -100,6 +100,12 @@ def _load() -> C.CDLL:
         "mgpm_last_error": (C.c_char_p, []),
         "mgpm_abi_version": (C.c_int, []),
         "mgpm_graph_load": (C.c_int, [C.c_char_p, pp]),
+        "mgpm_graph_load_ex": (C.c_int, [C.c_char_p, i32, pp]),
+        "mgpm_graph_labels": (C.c_int, [P, pp, C.POINTER(i32), pp,
+                                        C.POINTER(i32)]),
+        "mgpm_graph_set_labels": (C.c_int, [P, i32, P, i32, P]),
+        "mgpm_graph_label_frequency": (C.c_int, [P, P]),
+        "mgpm_graph_nlf": (C.c_int, [P, P, P, P]),
         "mgpm_graph_from_csr": (C.c_int, [i32, i64, P, P, pp]),
         "mgpm_graph_read_edges": (C.c_int, [C.c_char_p, i32, i32, pp]),
         "mgpm_graph_write": (C.c_int, [P, C.c_char_p]),
@@ -153,7 +159,8 @@ lib = _load()
 EXPORTED = [
     "mgpm_strerror", "mgpm_abi_version", "mgpm_last_error",
     "mgpm_graph_load", "mgpm_graph_from_csr", "mgpm_graph_read_edges",
-    "mgpm_graph_write",
+    "mgpm_graph_write", "mgpm_graph_load_ex", "mgpm_graph_labels",
+    "mgpm_graph_set_labels", "mgpm_graph_label_frequency", "mgpm_graph_nlf",
     "mgpm_graph_info_get", "mgpm_graph_csr", "mgpm_graph_free",
     "mgpm_graph_generate_rmat", "mgpm_graph_generate_er",
     "mgpm_graph_orientation", "mgpm_device_count", "mgpm_device_synchronize",
@@ -204,9 +211,14 @@ class Graph:
 
     # -- constructors ------------------------------------------------------
     @classmethod
-    def load(cls, prefix: str) -> "Graph":
+    def load(cls, prefix: str, vlabels: bool = False,
+             elabels: bool = False) -> "Graph":
+        """Reference format (graph.cc:6-92); with vlabels / elabels also
+        <prefix>.vlabel.bin / <prefix>.elabel.bin (graph.cc:37-82)."""
         h = C.c_void_p()
-        check(lib.mgpm_graph_load(str(prefix).encode(), C.byref(h)),
+        flags = (1 if vlabels else 0) | (2 if elabels else 0)
+        check(lib.mgpm_graph_load_ex(str(prefix).encode(), flags,
+                                     C.byref(h)),
               f"mgpm_graph_load({prefix})")
         return cls(h)
 
@@ -274,6 +286,62 @@ class Graph:
         else:
             colidx = np.zeros(0, dtype=np.int32)
         return rowptr, colidx
+
+    def labels(self) -> tuple:
+        """(vlabels uint8[n] or None, num_vertex_classes, elabels
+        uint16[nnz] or None, num_edge_classes), copies."""
+        vp, ep = C.c_void_p(), C.c_void_p()
+        nv, ne = C.c_int32(), C.c_int32()
+        check(lib.mgpm_graph_labels(self._h, C.byref(vp), C.byref(nv),
+                                    C.byref(ep), C.byref(ne)),
+              "mgpm_graph_labels")
+        gi = self.info()
+        v = e = None
+        if nv.value:
+            v = (np.ctypeslib.as_array(C.cast(vp, C.POINTER(C.c_uint8)),
+                                       shape=(gi.n,)).copy()
+                 if gi.n else np.zeros(0, np.uint8))
+        if ne.value:
+            e = (np.ctypeslib.as_array(C.cast(ep, C.POINTER(C.c_uint16)),
+                                       shape=(gi.nnz,)).copy()
+                 if gi.nnz else np.zeros(0, np.uint16))
+        return v, nv.value, e, ne.value
+
+    def set_labels(self, vlabels=None, num_vertex_classes: int = 0,
+                   elabels=None, num_edge_classes: int = 0) -> None:
+        v = None if vlabels is None else np.ascontiguousarray(vlabels,
+                                                              np.uint8)
+        e = None if elabels is None else np.ascontiguousarray(elabels,
+                                                              np.uint16)
+        if v is not None and len(v) != self.n:
+            raise ValueError("vlabels: one per vertex")
+        if e is not None and len(e) != self.nnz:
+            raise ValueError("elabels: one per CSR entry")
+        check(lib.mgpm_graph_set_labels(
+            self._h, num_vertex_classes if v is not None else 0,
+            None if v is None else _ptr(v),
+            num_edge_classes if e is not None else 0,
+            None if e is None else _ptr(e)), "mgpm_graph_set_labels")
+
+    def label_frequency(self) -> np.ndarray:
+        """Vertices per label 0..num_vertex_classes (graph.cc:440-457)."""
+        nv = self.labels()[1]
+        f = np.zeros(max(nv, 0) + 1, dtype=np.int64)
+        check(lib.mgpm_graph_label_frequency(self._h, _ptr(f)),
+              "mgpm_graph_label_frequency")
+        return f
+
+    def nlf(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """Neighbourhood label frequency of every vertex (graph.cc:483-495)
+        as (offsets int64[n+1], labels int32, counts int32)."""
+        off = np.zeros(self.n + 1, dtype=np.int64)
+        check(lib.mgpm_graph_nlf(self._h, _ptr(off), None, None),
+              "mgpm_graph_nlf")
+        lab = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        cnt = np.zeros(max(int(off[-1]), 1), dtype=np.int32)
+        check(lib.mgpm_graph_nlf(self._h, _ptr(off), _ptr(lab), _ptr(cnt)),
+              "mgpm_graph_nlf")
+        return off, lab[:off[-1]], cnt[:off[-1]]
 
     def write(self, prefix: str) -> None:
         check(lib.mgpm_graph_write(self._h, str(prefix).encode()),

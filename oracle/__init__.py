@@ -41,6 +41,8 @@ def lib() -> C.CDLL:
         L.orc_pattern.argtypes = [i32, P, P, i32, i32, P]
         L.orc_pattern_range.restype = C.c_uint64
         L.orc_pattern_range.argtypes = [i32, P, P, i32, i32, i32, i32, P]
+        L.orc_pattern_work.restype = C.c_uint64
+        L.orc_pattern_work.argtypes = [i32, P, P, i32, i32, P, P]
         L.orc_brute.restype = i32
         L.orc_brute.argtypes = [i32, P, P, P]
         _lib = L
@@ -139,6 +141,17 @@ def pattern(rowptr, colidx, p: int, threads: int = 0, v_lo: int = 0,
     r = lib().orc_pattern_range(n, _p(rp), _p(ci), p, v_lo,
                                 n if v_hi is None else v_hi, threads, _p(c))
     return int(r), int(c[0]), int(c[1])
+
+
+def pattern_work(rowptr, colidx, p: int, threads: int = 0) -> tuple[int, int]:
+    """Returns (combined count, set-op work W = sum of |a| + |b| over the
+    set operations of the reference's algebra)."""
+    rp, ci = _csr(rowptr, colidx)
+    c = np.zeros(2, dtype=np.uint64)
+    w = np.zeros(1, dtype=np.uint64)
+    r = lib().orc_pattern_work(len(rp) - 1, _p(rp), _p(ci), p, threads,
+                               _p(c), _p(w))
+    return int(r), int(w[0])
 
 
 def brute(rowptr, colidx) -> dict:

@@ -17,8 +17,15 @@
 
 /* ------------------------------------------------------------------ sets */
 
+/* Set-op work of the pattern passes, per thread: the sum over set
+ * operations of (|a| + |b|) (a membership test is a set operation with
+ * |a| = 1), the quantity the reference's PROFILE macro instruments
+ * (gpu_kernels/tuning_schedules.cuh:32-39). */
+static _Thread_local uint64_t g_work;
+
 /* exact membership in a sorted list */
 static int has(const int32_t *a, int32_t na, int32_t key) {
+  g_work += 1u + (uint64_t)na;
   int32_t lo = 0, hi = na;
   while (lo < hi) {
     int32_t mid = lo + ((hi - lo) >> 1);
@@ -32,6 +39,7 @@ static int has(const int32_t *a, int32_t na, int32_t key) {
  * set_difference.cuh:129-163 / 200-237 semantics. */
 static int32_t diff_ub(const int32_t *a, int32_t na, const int32_t *b,
                        int32_t nb, int32_t ub, int32_t *out) {
+  g_work += (uint64_t)na + (uint64_t)nb;
   int32_t i = 0, j = 0, k = 0;
   while (i < na && a[i] < ub) {
     while (j < nb && b[j] < a[i]) ++j;
@@ -47,6 +55,7 @@ static int32_t diff_ub(const int32_t *a, int32_t na, const int32_t *b,
 /* {x in a and b : x < ub} by merge.  set_intersect.cuh:73-193 semantics. */
 static int32_t inter_ub(const int32_t *a, int32_t na, const int32_t *b,
                         int32_t nb, int32_t ub, int32_t *out) {
+  g_work += (uint64_t)na + (uint64_t)nb;
   int32_t i = 0, j = 0, k = 0;
   while (i < na && j < nb) {
     if (a[i] >= ub || b[j] >= ub) break;
@@ -428,9 +437,27 @@ static void pattern_edge(const int64_t *rp, const int32_t *ci, int pat,
   }
 }
 
+static uint64_t pattern_range(int32_t n, const int64_t *rp, const int32_t *ci,
+                              int32_t pat, int32_t v_lo, int32_t v_hi,
+                              int32_t nthreads, uint64_t counters[2],
+                              uint64_t *work);
+
 uint64_t orc_pattern_range(int32_t n, const int64_t *rp, const int32_t *ci,
                            int32_t pat, int32_t v_lo, int32_t v_hi,
                            int32_t nthreads, uint64_t counters[2]) {
+  return pattern_range(n, rp, ci, pat, v_lo, v_hi, nthreads, counters, NULL);
+}
+
+uint64_t orc_pattern_work(int32_t n, const int64_t *rp, const int32_t *ci,
+                          int32_t pat, int32_t nthreads, uint64_t counters[2],
+                          uint64_t *work) {
+  return pattern_range(n, rp, ci, pat, 0, n, nthreads, counters, work);
+}
+
+static uint64_t pattern_range(int32_t n, const int64_t *rp, const int32_t *ci,
+                              int32_t pat, int32_t v_lo, int32_t v_hi,
+                              int32_t nthreads, uint64_t counters[2],
+                              uint64_t *work) {
   if (pat < 1 || pat > 8) return 0;
   if (v_lo < 0) v_lo = 0;
   if (v_hi > n) v_hi = n;
@@ -438,9 +465,10 @@ uint64_t orc_pattern_range(int32_t n, const int64_t *rp, const int32_t *ci,
   for (int32_t v = 0; v < n; ++v)
     if (DEG(v) > maxd) maxd = DEG(v);
   if (nthreads < 1) nthreads = omp_get_max_threads();
-  uint64_t c1 = 0, c2 = 0;
-#pragma omp parallel num_threads(nthreads) reduction(+ : c1, c2)
+  uint64_t c1 = 0, c2 = 0, w = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : c1, c2, w)
   {
+    g_work = 0;
     scratch_t s;
     for (int k = 0; k < 8; ++k)
       s.l[k] = (int32_t *)malloc((size_t)maxd * sizeof(int32_t));
@@ -458,11 +486,13 @@ uint64_t orc_pattern_range(int32_t n, const int64_t *rp, const int32_t *ci,
     }
     for (int k = 0; k < 8; ++k) free(s.l[k]);
     free(s.f);
+    w += g_work;
   }
   if (counters) {
     counters[0] = c1;
     counters[1] = c2;
   }
+  if (work) *work = w;
   /* sm_multigpu.cu:209-232 (one rank): P4, P6 -> c1; others c1 - c2 */
   if (pat == 4 || pat == 6) return c1;
   return c1 - c2;

@@ -70,6 +70,13 @@ uint64_t orc_pattern_range(int32_t n, const int64_t *rowptr,
                            const int32_t *colidx, int32_t pattern,
                            int32_t v_lo, int32_t v_hi, int32_t nthreads,
                            uint64_t counters[2]);
+/* orc_pattern with the set-op work W = sum over set operations of
+ * (|a| + |b|) (membership tests as |a| = 1) in *work: the quantity the
+ * reference's PROFILE macro instruments (tuning_schedules.cuh:32-39). */
+uint64_t orc_pattern_work(int32_t n, const int64_t *rowptr,
+                          const int32_t *colidx, int32_t pattern,
+                          int32_t nthreads, uint64_t counters[2],
+                          uint64_t *work);
 
 /* Exhaustive induced-subgraph census on n <= 64 vertices.  out[] gets:
  *  0 triangles            1 claw K1,3 (induced)   2 path P4 (induced)

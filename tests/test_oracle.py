@@ -221,3 +221,14 @@ def test_golden_counts_stable():
         for p in range(1, 9):
             assert oracle.pattern(rp, ci, p)[0] == g["counts"][f"P{p}"], \
                 (name, p)
+
+
+def test_pattern_work_known_answer():
+    """Set-op work of the reference's P4 algebra on the path 0-1-2, by hand:
+    edge (1,0): two differences of sizes 2+1 and 1+2, then membership tests
+    of L1 = {1} in N(0) and N(2) (1+1 each); edge (2,1): differences 1+2
+    and 2+1, then tests of {0,2} in N(1) (1+2 each): 22 in all."""
+    rp, ci = csr_from_edges(3, [(0, 1), (1, 2)])
+    count, work = oracle.pattern_work(rp, ci, 4)
+    assert count == oracle.pattern(rp, ci, 4)[0] == 0
+    assert work == 22
