@@ -208,8 +208,16 @@ def run_batch(ma, args) -> dict:
             if (k + 1) % 256 == 0:
                 log(f"batch: {k + 1}/{len(queries)} single queries done")
         return res
-    for _ in range(max(1, args.batch_warmup)):
-        single = all_single()
+    if args.batch_warmup > 0:
+        for _ in range(args.batch_warmup):
+            single = all_single()
+    else:
+        # light warm-up: one census query per graph (its census DAG is built
+        # and kept with the graph) and one query per pattern (kernels loaded)
+        for dg in dgs:
+            dg.pattern_count(1)
+        for p in pats:
+            one(dgs[0], p)
     sync()
     t_single = []
     for _ in range(args.batch_steps):
@@ -221,8 +229,14 @@ def run_batch(ma, args) -> dict:
     # the same queries in one batch call: one census per graph serves its
     # pattern queries, lanes overlap the graphs
     b = ma.Batch(0, lanes=args.batch_lanes, use_graph=args.batch_use_graph)
-    for _ in range(max(1, args.batch_warmup)):
-        counts, st = b.run(queries)
+    if args.batch_warmup > 0:
+        for _ in range(args.batch_warmup):
+            counts, st = b.run(queries)
+    else:
+        # the lanes' workspaces grown once, by the queries of the largest
+        # graphs (one graph per lane)
+        big = sorted(range(len(dgs)), key=lambda k: -int(graphs[k].nnz))
+        b.run([(dgs[k], p) for k in big[:args.batch_lanes] for p in pats])
     sync()
     log("batch: warm-up batch done")
     t_batch, kms = [], []
@@ -234,6 +248,8 @@ def run_batch(ma, args) -> dict:
         kms.append(st.kernel_ms)
         log(f"batch: one batch {t_batch[-1]:.2f} s")
     b.close()
+    for dg in dgs:
+        dg.close()
     # the oracle on the small graphs (its pattern algebra is cubic)
     checked, oracle_ok = 0, True
     for k, ((s_, ef), g) in enumerate(zip(specs, graphs)):
@@ -364,7 +380,8 @@ def bench_patterns(ma, args) -> int:
     return 0 if agree else 1
 
 
-def run_serve(ma, args, rate: float) -> dict:
+def run_serve(ma, args, rate: float, n: int, single: bool = True,
+              graphs_specs=None) -> dict:
     """The served stream: Poisson arrivals at `rate` req/s over resident
     RMAT graphs (default 64, scale uniform in [12, 20], ef 16, seed 6),
     pattern uniform over TC, P1..P8, through the continuous-batching server
@@ -377,11 +394,13 @@ def run_serve(ma, args, rate: float) -> dict:
     rng = np.random.default_rng(6)
     specs = [int(rng.integers(lo, hi + 1)) for _ in range(args.serve_graphs)]
     t0 = time.perf_counter()
-    graphs = [ma.Graph.rmat(s, 16, seed=6 * 100003 + i)
-              for i, s in enumerate(specs)]
+    if graphs_specs is None:
+        graphs = [ma.Graph.rmat(s, 16, seed=6 * 100003 + i)
+                  for i, s in enumerate(specs)]
+    else:
+        graphs = graphs_specs
     gen_s = time.perf_counter() - t0
     log(f"serve: {len(graphs)} graphs generated ({gen_s:.1f} s)")
-    n = args.serve_requests
     arrivals = np.cumsum(rng.exponential(1e6 / rate, size=n))
     req = [(int(rng.integers(len(graphs))), pats[int(rng.integers(len(pats)))])
            for _ in range(n)]
@@ -435,7 +454,11 @@ def run_serve(ma, args, rate: float) -> dict:
                 "busy_ms": st.busy_ms, "deferred": int(st.deferred),
                 "add_graphs_s": add_s}, [r.count for r in out]
 
-    served, counts = run(args.serve_slots, 1024, args.serve_lanes, "batched")
+    served, counts = run(args.serve_slots, 1024, args.serve_lanes,
+                         f"batched at {rate:g} req/s")
+    if not single:
+        return {"rate": rate, "requests": n, **served,
+                "counts": counts, "graphs": graphs}
     single, counts1 = run(1, 1, 1, "one launch per request")
     # each result is the same alone: one single-query call per distinct
     # (graph, pattern) among the first requests
@@ -465,12 +488,55 @@ def run_serve(ma, args, rate: float) -> dict:
         "qps_speedup": served["qps"] / single["qps"],
         "counts_equal_modes": counts == counts1,
         "alone_checked": len(seen), "counts_equal_alone": alone_ok,
-        "graph_gen_s": gen_s,
+        "graph_gen_s": gen_s, "graphs": graphs,
     }
 
 
+def _guarded(what, fn, *a):
+    """fn(*a), or {"error": ...} if it raised: a failing extra measurement
+    must not take the triangle-count line with it."""
+    try:
+        return fn(*a)
+    except Exception as e:  # noqa: BLE001
+        log(f"{what} failed: {e!r}")
+        return {"error": repr(e)}
+
+
+def run_extras(ma, args) -> dict:
+    """The other measurements of the default line, each a sub-object: the
+    small CPU baselines, the batch workload (512 graphs, TC+P2+P4) and the
+    served stream (at the offered rate, with its one-launch-per-request
+    comparison, and at a rate below saturation)."""
+    import copy
+    extras = [e.strip() for e in args.extras.split(",") if e.strip()]
+    out = {}
+    if "cpu" in extras:
+        out["cpu_small_baselines"] = _guarded("cpu baselines",
+                                              cpu_small_baselines, ma)
+    if "batch" in extras:
+        a = copy.copy(args)
+        a.batch_warmup = 0
+        out["batch"] = _guarded("batch", run_batch, ma, a)
+    if "serve" in extras:
+        def serve():
+            a = copy.copy(args)
+            a.serve_alone_checks = args.extras_alone_checks
+            r = run_serve(ma, a, args.serve_rate, args.extras_serve_requests)
+            graphs = r.pop("graphs")
+            rate = args.serve_low_rate or round(0.4 * r["served"]["qps"], 2)
+            low = run_serve(ma, a, rate, args.serve_low_requests,
+                            single=False, graphs_specs=graphs)
+            low.pop("graphs")
+            low.pop("counts")
+            r["below_saturation"] = low
+            return r
+        out["serve"] = _guarded("serve", serve)
+    return out
+
+
 def bench_serve(ma, args) -> int:
-    r = run_serve(ma, args, args.serve_rate)
+    r = run_serve(ma, args, args.serve_rate, args.serve_requests)
+    r.pop("graphs")
     out = {
         "metric": "served queries/s", "value": r["qps"], "unit": "queries/s",
         "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -496,6 +562,10 @@ def main() -> int:
     ap.add_argument("--patterns", default="P1,P4,P7,P8")
     ap.add_argument("--pattern-methods", default="census",
                     help="patterns mode: census and/or kernels")
+    ap.add_argument("--tc-layout", type=int, default=0,
+                    help="tc mode: 1 = stream laid-out partner lists, -1 = "
+                         "gather them from the DAG, 0 = the library's "
+                         "default (lay out when the copy fits)")
     ap.add_argument("--no-device-plan", action="store_true",
                     help="skip the host-plan vs device-plan upload timing")
     ap.add_argument("--mode", default="tc",
@@ -525,6 +595,16 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-patterns", default="TC,P2,P4")
+    ap.add_argument("--extras", default="cpu,batch,serve",
+                    help="tc mode: sub-objects added to the line (cpu: the "
+                         "small CPU baselines; batch: the batch workload; "
+                         "serve: the served stream); '' for none")
+    ap.add_argument("--extras-serve-requests", type=int, default=80)
+    ap.add_argument("--extras-alone-checks", type=int, default=20)
+    ap.add_argument("--serve-low-rate", type=float, default=0.0,
+                    help="below-saturation rate (0: 0.4 x the batched "
+                         "throughput at the offered rate)")
+    ap.add_argument("--serve-low-requests", type=int, default=20)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB "
                          "(5 times) to calibrate rocprofv3 FETCH_SIZE / "
@@ -538,7 +618,8 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    if args.mode in ("batch", "serve") and args.hw_queues > 0:
+    if (args.mode in ("batch", "serve") or
+            (args.mode == "tc" and args.extras.strip())) and args.hw_queues > 0:
         # Read by the HIP runtime when it starts (before any HIP call): how
         # many hardware queues this process's streams may spread over, so
         # how many lanes of a batch can run on the device at once.
@@ -562,14 +643,18 @@ def main() -> int:
     t0 = time.perf_counter()
     dg = ma.DeviceGraph(g, 0)
     upload_s = time.perf_counter() - t0
+    tune = ma.make_tune(tc_layout=args.tc_layout) if args.tc_layout else None
+    # plan preparation outside the warm-up and timed steps: the first count
+    # lays the partner lists out (when it does), reported as prep
+    prep_ms = dg.tc_count(tune)[1].prep_ms
     for _ in range(args.warmup):
-        dg.tc_count()
+        dg.tc_count(tune)
     ma.native.check(ma.native.lib.mgpm_device_synchronize(0), "sync")
     kernel_ms = []
     counts = set()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        c, st = dg.tc_count()
+        c, st = dg.tc_count(tune)
         kernel_ms.append(st.kernel_ms)
         counts.add(c)
     ma.native.check(ma.native.lib.mgpm_device_synchronize(0), "sync")
@@ -610,6 +695,8 @@ def main() -> int:
         },
         "count": count,
         "probes": st.probes,
+        "laid_out_lists": st.path == 2,
+        "layout_build_ms": prep_ms,
         "kernel_edges_per_s": st.m / (avg_kernel_ms / 1e3),
         "gen_s": gen_s,
         "upload_prep_s": upload_s,
@@ -618,6 +705,9 @@ def main() -> int:
         out.update(upload_compare(ma, g, count))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, count, st.m)
+    dg.close()
+    if args.extras.strip():
+        out.update(run_extras(ma, args))
     print(json.dumps(out), flush=True)
     return 0
 

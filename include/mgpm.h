@@ -103,6 +103,17 @@ typedef struct mgpm_tune {
                              5-cliques; for P2, P3, P5, P6 also wheels,
                              K2,3 and K2,3 plus an edge); -1 = the
                              reference's per-edge set algebra (same count) */
+  int32_t tc_layout;      /* TC (mgpm_tc_count): 1 = stream every task's
+                             partner lists from a per-graph copy laid out
+                             task by task (built on first use and kept with
+                             the plan: 4 bytes per probe + 8 per partner),
+                             -1 = gather them from the DAG; 0 (default) =
+                             lay out when the copy fits a quarter of the
+                             free device memory.  Same count either way. */
+  int32_t tc_merge_max;   /* TC: owners with d+ up to this (at most 64)
+                             skip the hash table: each lane merges one
+                             partner list with the owner's sorted list (two
+                             cursors); 0 (default) = never, -1 = never */
 } mgpm_tune;
 
 /* Per-call measurements. */

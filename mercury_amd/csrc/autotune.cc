@@ -34,6 +34,8 @@ mgpm_tune tune_of(const TcResident &r) {
   t.tc_items = r.items;
   t.grid_blocks = r.grid_cap;
   t.tc_cursor_min = r.cursor_min;
+  t.tc_layout = r.layout;
+  t.tc_merge_max = r.merge_max == 0 ? -1 : r.merge_max;
   return t;
 }
 
@@ -81,6 +83,8 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
   int rc = time_tc(dg, cur, reps, &want, &cur_ms);
   if (rc) return rc;
   const std::vector<Knob> knobs = {
+      {&mgpm_tune::tc_layout, {-1, 0}},
+      {&mgpm_tune::tc_merge_max, {-1, 16, 32, 64}},
       {&mgpm_tune::tc_items, {4, 8}},
       {&mgpm_tune::tc_cursor_min, {-1, 16, 32, 64}},
       {&mgpm_tune::small_max_deg, {32, 64, 128}},
@@ -112,6 +116,8 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
   dg->tc.items = cur.tc_items;
   dg->tc.grid_cap = cur.grid_blocks;
   dg->tc.cursor_min = cur.tc_cursor_min;
+  dg->tc.layout = cur.tc_layout;
+  dg->tc.merge_max = cur.tc_merge_max > 0 ? cur.tc_merge_max : 0;
   dg->tc.tuned = true;
   *best = cur;
   *best_ms = cur_ms;

@@ -35,7 +35,7 @@ int hip_fail(hipError_t e, const char *what) {
 }
 
 size_t TcResident::bytes() const {
-  size_t b = col.bytes + partners.bytes;
+  size_t b = col.bytes + partners.bytes + laid.bytes + laid_off.bytes;
   for (int c = 0; c < kTcNumClasses; ++c)
     b += tasks[c].bytes + groups[c].bytes;
   return b;
@@ -123,6 +123,8 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
   r.params = params;
   r.on_device = false;
   r.generation = next_generation();
+  r.drop_layout();
+  r.npart = plan.partners.size();
   r.s_plus = plan.s_plus;
   r.probes = plan.probes;
   r.m_plus = static_cast<uint64_t>(plan.m_plus);
@@ -134,6 +136,38 @@ int tc_make_resident(mgpm_dgraph *dg, const TcPlanParams &params,
 int tc_replan(mgpm_dgraph *dg, const TcPlanParams &params, hipStream_t s) {
   return dg->tc.on_device ? tc_make_resident_device(dg, params, s)
                           : tc_make_resident(dg, params, s);
+}
+
+// Bytes of the laid-out partner lists of the resident plan.
+uint64_t tc_layout_bytes(const TcResident &r) {
+  return 4ull * r.probes + 8ull * (r.npart + 1);
+}
+
+// Builds the laid-out partner lists of the resident plan (tc_layout) if
+// they are not there.
+int tc_layout_build(mgpm_dgraph *dg, hipStream_t s) {
+  TcResident &r = dg->tc;
+  if (r.laid.p && r.laid_off.p) return MGPM_OK;
+  DevBuf items, item_off, lens, tmp;
+  const size_t tb = tc_layout_scan_bytes(r.npart);
+  if (tb == 0) return fail(MGPM_ERR_HIP, "scan sizing failed");
+  MGPM_HIP(items.alloc(std::max<uint64_t>(r.probes, 1) * 4));
+  MGPM_HIP(item_off.alloc((r.npart + 1) * 8));
+  MGPM_HIP(lens.alloc((r.npart + 1) * 8));
+  MGPM_HIP(tmp.alloc(tb));
+  if (tc_layout(r.dev.col, r.dev.partners, r.npart, lens.as<uint64_t>(),
+                item_off.as<uint64_t>(), items.as<int32_t>(), tmp.p, tb,
+                dg->cus, s))
+    return fail(MGPM_ERR_HIP, "layout kernel launch failed");
+  uint64_t total = 0;
+  MGPM_HIP(hipMemcpyAsync(&total, item_off.as<uint64_t>() + r.npart, 8,
+                          hipMemcpyDeviceToHost, s));
+  MGPM_HIP(hipStreamSynchronize(s));
+  if (total != r.probes)
+    return fail(MGPM_ERR_INTERNAL, "laid-out lists: item count mismatch");
+  r.laid.swap(items);
+  r.laid_off.swap(item_off);
+  return MGPM_OK;
 }
 
 namespace {
@@ -540,8 +574,31 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
                 : dg->tc.items;
   l.cursor_min = (tune && tune->tc_cursor_min != 0) ? tune->tc_cursor_min
                                                     : dg->tc.cursor_min;
+  l.merge_max = (tune && tune->tc_merge_max != 0) ? tune->tc_merge_max
+                                                  : dg->tc.merge_max;
+  if (l.merge_max < 0) l.merge_max = 0;
+  // Laid-out partner lists: asked for (1), refused (-1), or by default
+  // when they fit a quarter of the free device memory.
+  const int lay = (tune && tune->tc_layout != 0) ? tune->tc_layout
+                                                 : dg->tc.layout;
+  bool use_layout = lay > 0;
+  if (lay == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (dg->tc.laid.p) use_layout = true;
+    else if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      use_layout = tc_layout_bytes(dg->tc) <= free_b / 4;
+  }
+  TcDevicePlan plan = dg->tc.dev;
+  if (use_layout) {
+    const auto tp = std::chrono::steady_clock::now();
+    const bool had = dg->tc.laid.p != nullptr;
+    if ((rc = tc_layout_build(dg, s))) return rc;
+    if (!had) prep += ms_since(tp);
+    plan.items = dg->tc.laid.as<int32_t>();
+    plan.item_off = dg->tc.laid_off.as<uint64_t>();
+  }
   MGPM_HIP(hipEventRecord(dg->ev0, s));
-  if (tc_device_launch(dg->tc.dev, l, s))
+  if (tc_device_launch(plan, l, s))
     return fail(MGPM_ERR_HIP, "triangle kernel launch failed");
   MGPM_HIP(hipEventRecord(dg->ev1, s));
   MGPM_HIP(hipMemcpyAsync(dg->pinned, dg->tc.dev.result, 8,
@@ -561,7 +618,7 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
     stats->probes = dg->tc.probes;
     stats->bytes_alg =
         8ull * (stats->n + 1) + 20ull * stats->m_plus + 4ull * stats->s_plus;
-    stats->path = 1;
+    stats->path = use_layout ? 2 : 1;
     stats->e2e_ms = ms_since(t0);
   }
   return MGPM_OK;

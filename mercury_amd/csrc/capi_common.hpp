@@ -80,10 +80,20 @@ struct TcResident {
   bool tuned = false;       // knobs chosen by the autotuner
   bool on_device = false;   // plan built by tc_make_resident_device
   uint64_t s_plus = 0, probes = 0, m_plus = 0;
+  uint64_t npart = 0;       // partners of the plan
   int32_t max_out = 0;
+  int layout = 0;           // tc_layout choice of the resident plan
+  int merge_max = 0;        // tc_merge_max choice (0: never)
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
+  // The plan's partner lists laid out task by task (tc_layout), built on
+  // first use and dropped with the plan.
+  DevBuf laid, laid_off;
   TcDevicePlan dev;
   size_t bytes() const;
+  void drop_layout() {
+    laid.reset();
+    laid_off.reset();
+  }
 };
 
 // Undirected CSR resident on one device (pattern kernels).

@@ -19,6 +19,11 @@ struct TcDevicePlan {
   const int32_t *col = nullptr;       // DAG column array
   uint32_t ncol = 0;                  // its length (m+)
   const TcPartner *partners = nullptr;
+  // Optional layout of the partner lists task by task (tc_layout): when
+  // both are set the kernels stream items[item_off[t.pbeg] ..
+  // item_off[t.pbeg + t.pcnt]) for task t instead of gathering its lists.
+  const int32_t *items = nullptr;
+  const uint64_t *item_off = nullptr;
   const TcTask *tasks[kTcNumClasses] = {nullptr, nullptr, nullptr, nullptr};
   int ntasks[kTcNumClasses] = {0, 0, 0, 0};
   const int32_t *groups[kTcNumClasses] = {nullptr, nullptr, nullptr, nullptr};
@@ -35,6 +40,8 @@ struct TcLaunch {
   bool reset = true;  // zero queues and result first (else the caller has)
   int cursor_min = -1;  // owner-cursor streaming from this average list
                         // length of a 64-partner batch (<= 0: never)
+  int merge_max = 0;    // owners of d+ up to this (at most 64) merge their
+                        // partner lists lane by lane instead of hashing
 };
 
 // Enqueues reset + counting kernels on `stream`; the count lands in
@@ -48,5 +55,15 @@ int device_zero(void *p, size_t words, hipStream_t stream);
 
 // One-time per-process kernel attributes (large dynamic LDS).
 int tc_device_init();
+
+// The partner lists laid out in partner order, so that each task's items
+// are contiguous: item_off[0..np] = exclusive scan of partners[].len (lens,
+// np + 1 words, is scratch), items[item_off[i] .. item_off[i + 1]) = the
+// list of partner i.  `items` holds item_off[np] words; scan_tmp holds
+// tc_layout_scan_bytes(np) bytes.  Returns 0, or -1 if a launch failed.
+size_t tc_layout_scan_bytes(uint64_t np);
+int tc_layout(const int32_t *col, const TcPartner *partners, uint64_t np,
+              uint64_t *lens, uint64_t *item_off, int32_t *items,
+              void *scan_tmp, size_t scan_bytes, int cus, hipStream_t stream);
 
 }  // namespace mgpm

@@ -155,6 +155,41 @@ __device__ __forceinline__ uint32_t stream_partners(
   return hits;
 }
 
+// Streams the laid-out items [ib, ie) of one task (its partner lists,
+// concatenated in partner order) through `probe`, 64 * ITEMS items per step;
+// wave w of nw takes steps w, w + nw, ...  ib and ie are wave-uniform, so
+// each load is one scalar base plus a lane offset.  Returns this lane's
+// hit count.
+template <int ITEMS, class Probe>
+__device__ __forceinline__ uint32_t stream_items(
+    const int32_t *__restrict__ items, uint64_t ib, uint64_t ie,
+    const Probe &probe, int w, int nw) {
+  const int lane = lane64();
+  constexpr uint64_t kStep = 64ull * ITEMS;
+  uint32_t hits = 0;
+  for (uint64_t k = ib + uint64_t(w) * kStep; k < ie; k += uint64_t(nw) * kStep) {
+    const int32_t *base = items + k;
+    int32_t v[ITEMS];
+    bool ok[ITEMS];
+    if (k + kStep <= ie) {  // a full step: no guards
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        v[i] = base[64 * i + lane];
+        ok[i] = true;
+      }
+    } else {
+      const int left = static_cast<int>(ie - k);
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        ok[i] = 64 * i + lane < left;
+        v[i] = ok[i] ? base[64 * i + lane] : -2;
+      }
+    }
+    hits += probe.template lookup<ITEMS>(v, ok);
+  }
+  return hits;
+}
+
 // Open-addressing hash set in LDS, empty = -1.  Keys are vertex ids (>= 0),
 // so an idle item (w = -2) never matches and its probe ends at the first
 // empty slot (tables are at most half full).
@@ -277,16 +312,23 @@ __device__ __forceinline__ void flush_hits(unsigned long long acc,
 struct TaskPre {
   TcPartner first{0u, 0u};
   int32_t key0 = 0, key1 = 0;
+  uint64_t ib = 0, ie = 0;  // laid-out items of the task (STREAM)
 };
 
 constexpr int kTaskPreMaxTlen = 128;
 
 // Tasks have pcnt >= 1 and tlen >= 1 (owners without partners get none).
+template <bool STREAM = false>
 __device__ __forceinline__ TaskPre load_task_pre(
     const int32_t *__restrict__ col, const TcPartner *__restrict__ partners,
-    const TcTask &t, int lane) {
+    const TcTask &t, int lane, const uint64_t *__restrict__ item_off = nullptr) {
   TaskPre p;
-  p.first = partners[t.pbeg + static_cast<uint32_t>(min(lane, t.pcnt - 1))];
+  if constexpr (STREAM) {
+    p.ib = item_off[t.pbeg];
+    p.ie = item_off[t.pbeg + static_cast<uint32_t>(t.pcnt)];
+  } else {
+    p.first = partners[t.pbeg + static_cast<uint32_t>(min(lane, t.pcnt - 1))];
+  }
   p.key0 = col[t.tbeg + static_cast<uint32_t>(min(lane, t.tlen - 1))];
   p.key1 = col[t.tbeg + static_cast<uint32_t>(min(lane + 64, t.tlen - 1))];
   return p;
@@ -356,6 +398,96 @@ __device__ __forceinline__ uint32_t wave_task(const int32_t *__restrict__ col,
   const uint32_t hits = stream_partners<ITEMS>(
       col, ncol, partners, task.pbeg, task.pcnt, 0, 1, mark, probe, cursor_min,
       &pre.first);
+  wave_sync();
+  return hits;
+}
+
+// Owners of up to kMergeMaxTlen items can skip the table: the owner list
+// goes to LDS as it is (sorted), and each lane merges one partner list
+// (sorted, at most as long as the owner's by the owner rule) against it
+// with two cursors, the next partner item loaded one step ahead.  Lists
+// come from the DAG (partners) or, when `items` is set, from the layout.
+constexpr int kMergeMaxTlen = 64;
+
+__device__ __forceinline__ uint32_t wave_task_merge(
+    const int32_t *__restrict__ col, const TcPartner *__restrict__ partners,
+    const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
+    const TcTask &task, int32_t *own_lds, const TaskPre &pre) {
+  const int lane = lane64();
+  lds_i32 *own = as_lds(own_lds);
+  const int tlen = task.tlen;  // 1 .. kMergeMaxTlen
+  if (lane < tlen) own[lane] = pre.key0;
+  wave_sync();
+  uint32_t hits = 0;
+  for (int pb = 0; pb < task.pcnt; pb += 64) {
+    const int pi = pb + lane;
+    if (pi >= task.pcnt) continue;
+    const int32_t *L;
+    int len;
+    const uint32_t p = task.pbeg + static_cast<uint32_t>(pi);
+    if (items) {
+      const uint64_t b0 = item_off[p];
+      L = items + b0;
+      len = static_cast<int>(item_off[p + 1] - b0);
+    } else {
+      const TcPartner pr = partners[p];
+      L = col + pr.beg;
+      len = static_cast<int>(pr.len);
+    }
+    if (len <= 0) continue;
+    int i = 0, j = 0;
+    int32_t a = own[0];
+    int32_t b = L[0];
+    int32_t bn = len > 1 ? L[1] : 0;
+    for (;;) {
+      const bool ai = a <= b, bj = b <= a;  // both on a hit
+      hits += (ai && bj) ? 1u : 0u;
+      if (ai && ++i >= tlen) break;
+      if (bj && ++j >= len) break;
+      if (ai) a = own[i];
+      if (bj) {
+        b = bn;
+        bn = j + 1 < len ? L[j + 1] : 0;
+      }
+    }
+  }
+  wave_sync();  // the owner list is rewritten by the next task
+  return hits;
+}
+
+// wave_task over the task's laid-out items (pre.ib .. pre.ie) instead of
+// its partner lists.
+template <int ITEMS>
+__device__ __forceinline__ uint32_t wave_task_stream(
+    const int32_t *__restrict__ col, const int32_t *__restrict__ items,
+    const TcTask &task, int32_t *table, int logp_max, const TaskPre &pre) {
+  const int lane = lane64();
+  const uint64_t ib = wave_first_u64(pre.ib);
+  const uint64_t ie = wave_first_u64(pre.ie);
+  if (2 * task.tlen > (1 << logp_max)) {
+    SortedProbe probe{col + task.tbeg, task.tlen};
+    return stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
+  }
+  int logp = ceil_log2(8 * task.tlen);
+  logp = logp < 6 ? 6 : (logp > logp_max ? logp_max : logp);
+  const int P = 1 << logp;
+  for (int i = lane; i < P; i += 64) table[i] = -1;
+  wave_sync();
+  if (task.tlen <= kTaskPreMaxTlen)
+    table_insert_keys(table, logp, pre.key0, lane < task.tlen, pre.key1,
+                      lane + 64 < task.tlen, logp_max > 8);
+  else
+    table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64,
+                 logp_max > 8);
+  wave_sync();
+  uint32_t hits;
+  if (logp_max <= 8) {
+    HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
+    hits = stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
+  } else {
+    HashProbe<true> probe{table, logp, static_cast<uint32_t>(P - 1)};
+    hits = stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
+  }
   wave_sync();
   return hits;
 }

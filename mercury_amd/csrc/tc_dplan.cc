@@ -193,6 +193,8 @@ int tc_make_resident_device(mgpm_dgraph *dg, const TcPlanParams &params,
   r.dev = p;
   r.params = params;
   r.generation = next_generation();
+  r.drop_layout();
+  r.npart = static_cast<uint64_t>(np);
   r.s_plus = sum.s_plus;
   r.probes = static_cast<uint64_t>(cost_total - kTcPartnerCost * np);
   r.m_plus = static_cast<uint64_t>(m);

@@ -20,6 +20,8 @@
 //     critical path without putting several heavy tasks on one wave.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 
 #include "tc_device_fns.hpp"
@@ -31,13 +33,16 @@ namespace {
 using namespace tcdev;
 
 // Wave-private tables of up to 2^LOGP_MAX slots (tasks with 2*d+ <= that).
-template <int LOGP_MAX, int WAVES, int ITEMS>
+// STREAM: every task streams its laid-out items (items[item_off[pbeg] ..
+// item_off[pbeg + pcnt])) instead of gathering its partner lists.
+template <int LOGP_MAX, int WAVES, int ITEMS, bool STREAM>
 __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
+    const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
-    int cursor_min) {
+    int cursor_min, int merge_max) {
   __shared__ int32_t s_table[WAVES][1 << LOGP_MAX];
   __shared__ int32_t s_mark[WAVES][64 * ITEMS];
   const int lane = lane64();
@@ -52,14 +57,22 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     const int t_end = goff[g + 1];
     int t = goff[g];
     TcTask task = tasks[t];
-    TaskPre pre = load_task_pre(col, partners, task, lane);
+    TaskPre pre = load_task_pre<STREAM>(col, partners, task, lane, item_off);
     for (; t < t_end; ++t) {
       // The next task's loads go out before this task's gathers (the last
       // task of the group reloads itself: same load count on every path).
       const TcTask next = tasks[t + 1 < t_end ? t + 1 : t];
-      const TaskPre next_pre = load_task_pre(col, partners, next, lane);
-      acc += wave_task<ITEMS>(col, ncol, partners, task, table, LOGP_MAX,
-                              s_mark[wv], cursor_min, pre);
+      const TaskPre next_pre =
+          load_task_pre<STREAM>(col, partners, next, lane, item_off);
+      if (task.tlen <= merge_max)
+        acc += wave_task_merge(col, partners, STREAM ? items : nullptr,
+                               item_off, task, table, pre);
+      else if constexpr (STREAM)
+        acc += wave_task_stream<ITEMS>(col, items, task, table, LOGP_MAX,
+                                       pre);
+      else
+        acc += wave_task<ITEMS>(col, ncol, partners, task, table, LOGP_MAX,
+                                s_mark[wv], cursor_min, pre);
       task = next;
       pre = next_pre;
     }
@@ -68,11 +81,12 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
 }
 
 // One table per workgroup (dynamic LDS of 2^logp_max slots).
-template <int WAVES, int ITEMS>
+template <int WAVES, int ITEMS, bool STREAM>
 __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
+    const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int logp_max, int cursor_min) {
   extern __shared__ int32_t s_table[];
@@ -88,10 +102,19 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     const int t_end = goff[g + 1];
     for (int t = goff[g]; t < t_end; ++t) {
     const TcTask task = tasks[t];
+    uint64_t ib = 0, ie = 0;
+    if constexpr (STREAM) {
+      ib = item_off[task.pbeg];
+      ie = item_off[task.pbeg + static_cast<uint32_t>(task.pcnt)];
+    }
     if (2 * task.tlen > (1 << logp_max)) {  // never planned; kept exact
       SortedProbe probe{col + task.tbeg, task.tlen};
-      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
-                             WAVES, s_mark[wv], probe, cursor_min);
+      if constexpr (STREAM)
+        acc += stream_items<ITEMS>(items, ib, ie, probe, wv, WAVES);
+      else
+        acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg,
+                                      task.pcnt, wv, WAVES, s_mark[wv], probe,
+                                      cursor_min);
       __syncthreads();
       continue;
     }
@@ -104,8 +127,11 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
                  WAVES * 64, true);
     __syncthreads();
     HashProbe<true> probe{s_table, logp, static_cast<uint32_t>(P - 1)};
-    acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, wv,
-                           WAVES, s_mark[wv], probe, cursor_min);
+    if constexpr (STREAM)
+      acc += stream_items<ITEMS>(items, ib, ie, probe, wv, WAVES);
+    else
+      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt,
+                                    wv, WAVES, s_mark[wv], probe, cursor_min);
     __syncthreads();
     }
   }
@@ -113,11 +139,12 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
 }
 
 // Tables too large for LDS: binary search in the global sorted out-list.
-template <int WAVES, int ITEMS>
+template <int WAVES, int ITEMS, bool STREAM>
 __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
     const int32_t *__restrict__ col, uint32_t ncol,
     const TcTask *__restrict__ tasks, const int32_t *__restrict__ goff,
     int ngroups, const TcPartner *__restrict__ partners,
+    const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int cursor_min) {
   __shared__ int32_t s_mark[WAVES][64 * ITEMS];
@@ -133,8 +160,15 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
     for (int t = goff[g]; t < t_end; ++t) {
       const TcTask task = tasks[t];
       SortedProbe probe{col + task.tbeg, task.tlen};
-      acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg, task.pcnt, 0, 1,
-                             s_mark[wv], probe, cursor_min);
+      if constexpr (STREAM)
+        acc += stream_items<ITEMS>(
+            items, item_off[task.pbeg],
+            item_off[task.pbeg + static_cast<uint32_t>(task.pcnt)], probe, 0,
+            1);
+      else
+        acc += stream_partners<ITEMS>(col, ncol, partners, task.pbeg,
+                                      task.pcnt, 0, 1, s_mark[wv], probe,
+                                      cursor_min);
     }
   }
   flush_hits(acc, result);
@@ -169,54 +203,109 @@ int grid_for(const void *kernel, int threads, size_t dyn_lds, int cus,
 
 int waves_to_blocks(int groups, int waves) { return (groups + waves - 1) / waves; }
 
-template <int ITEMS>
+template <int ITEMS, bool STREAM>
 int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
+  const int merge_max =
+      l.merge_max < kMergeMaxTlen ? l.merge_max : kMergeMaxTlen;
   // Heaviest classes first so their long tasks start early.
   if (p.ngroups[kTcBlock] > 0) {
     int logp = 6;
     while ((1 << logp) < 2 * p.class_max_tlen[kTcBlock]) ++logp;
     if (logp > kTcBlockLogpMax) return -1;
     const size_t lds = sizeof(int32_t) << logp;
-    auto *kern = &tc_block_tables<kBlockWaves, ITEMS>;
+    auto *kern = &tc_block_tables<kBlockWaves, ITEMS, STREAM>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
                               kBlockWaves * 64, lds, l.cus, l.grid_cap,
                               p.ngroups[kTcBlock]);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlockWaves * 64), lds, stream,
                        p.col, p.ncol, p.tasks[kTcBlock], p.groups[kTcBlock],
-                       p.ngroups[kTcBlock], p.partners, p.queues + kTcBlock,
+                       p.ngroups[kTcBlock], p.partners, p.items, p.item_off,
+                       p.queues + kTcBlock,
                        p.result, logp, l.cursor_min);
   }
   if (p.ngroups[kTcMedium] > 0) {
-    auto *kern = &tc_wave_tables<kMediumLogp, kWaves, ITEMS>;
+    auto *kern = &tc_wave_tables<kMediumLogp, kWaves, ITEMS, STREAM>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
                               kWaves * 64, 0, l.cus, l.grid_cap,
                               waves_to_blocks(p.ngroups[kTcMedium], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcMedium], p.groups[kTcMedium],
-                       p.ngroups[kTcMedium], p.partners,
-                       p.queues + kTcMedium, p.result, l.cursor_min);
+                       p.ngroups[kTcMedium], p.partners, p.items,
+                       p.item_off, p.queues + kTcMedium, p.result,
+                       l.cursor_min, merge_max);
   }
   if (p.ngroups[kTcSmall] > 0) {
-    auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS>;
+    auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS, STREAM>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
                               kWaves * 64, 0, l.cus, l.grid_cap,
                               waves_to_blocks(p.ngroups[kTcSmall], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcSmall], p.groups[kTcSmall],
-                       p.ngroups[kTcSmall], p.partners, p.queues + kTcSmall,
-                       p.result, l.cursor_min);
+                       p.ngroups[kTcSmall], p.partners, p.items, p.item_off,
+                       p.queues + kTcSmall, p.result, l.cursor_min,
+                       merge_max);
   }
   if (p.ngroups[kTcGlobal] > 0) {
-    auto *kern = &tc_global_search<kWaves, ITEMS>;
+    auto *kern = &tc_global_search<kWaves, ITEMS, STREAM>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
                               kWaves * 64, 0, l.cus, l.grid_cap,
                               waves_to_blocks(p.ngroups[kTcGlobal], kWaves));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
                        p.ncol, p.tasks[kTcGlobal], p.groups[kTcGlobal],
-                       p.ngroups[kTcGlobal], p.partners, p.queues + kTcGlobal,
-                       p.result, l.cursor_min);
+                       p.ngroups[kTcGlobal], p.partners, p.items, p.item_off,
+                       p.queues + kTcGlobal, p.result, l.cursor_min);
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void partner_lens(const TcPartner *__restrict__ partners,
+                             uint64_t np, uint64_t *__restrict__ lens) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i <= np;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    lens[i] = i < np ? partners[i].len : 0u;
+}
+
+// One wave per batch of 64 partners: item g of the batch (its lists
+// concatenated) belongs to the first partner o whose inclusive length sum
+// exceeds g, and goes to items[item_off[first] + g].
+constexpr int kLayoutWaves = 4;
+__global__ __launch_bounds__(kLayoutWaves * 64) void layout_lists(
+    const int32_t *__restrict__ col, const TcPartner *__restrict__ partners,
+    uint64_t np, const uint64_t *__restrict__ item_off,
+    int32_t *__restrict__ items) {
+  __shared__ int32_t s_incl[kLayoutWaves][64];
+  __shared__ uint32_t s_beg[kLayoutWaves][64];
+  const int lane = lane64();
+  const int wv = threadIdx.x >> 6;
+  const uint64_t nb = (np + 63) / 64;
+  for (uint64_t b = uint64_t(blockIdx.x) * kLayoutWaves + wv; b < nb;
+       b += uint64_t(gridDim.x) * kLayoutWaves) {
+    const uint64_t i = b * 64 + lane;
+    uint32_t beg = 0;
+    int len = 0;
+    if (i < np) {
+      const TcPartner pr = partners[i];
+      beg = pr.beg;
+      len = static_cast<int>(pr.len);
+    }
+    const int incl = wave_incl_scan(len);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
+    s_incl[wv][lane] = incl;
+    s_beg[wv][lane] = beg;
+    wave_sync();
+    int32_t *out = items + item_off[b * 64];
+    for (int g = lane; g < total; g += 64) {
+      int lo = 0, hi = 63;  // incl[63] = total > g
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_incl[wv][mid] > g) hi = mid;
+        else lo = mid + 1;
+      }
+      const int ex = lo > 0 ? s_incl[wv][lo - 1] : 0;
+      out[g] = col[s_beg[wv][lo] + static_cast<uint32_t>(g - ex)];
+    }
+    wave_sync();  // before the next batch overwrites the sums
+  }
 }
 
 __global__ void zero_words(uint32_t *p, size_t n) {
@@ -237,14 +326,47 @@ int device_zero(void *p, size_t words, hipStream_t stream) {
 
 int tc_device_init() {
   const int lds = static_cast<int>(sizeof(int32_t) << kTcBlockLogpMax);
-  hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 4>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 8>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  return e == hipSuccess ? 0 : -1;
+  const void *kerns[] = {
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 4, false>),
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 8, false>),
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 4, true>),
+      reinterpret_cast<const void *>(&tc_block_tables<kBlockWaves, 8, true>)};
+  for (const void *k : kerns)
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds) != hipSuccess)
+      return -1;
+  return 0;
+}
+
+size_t tc_layout_scan_bytes(uint64_t np) {
+  size_t b = 0;
+  if (hipcub::DeviceScan::ExclusiveSum(
+          nullptr, b, static_cast<const uint64_t *>(nullptr),
+          static_cast<uint64_t *>(nullptr), np + 1) != hipSuccess)
+    return 0;
+  return b;
+}
+
+int tc_layout(const int32_t *col, const TcPartner *partners, uint64_t np,
+              uint64_t *lens, uint64_t *item_off, int32_t *items,
+              void *scan_tmp, size_t scan_bytes, int cus, hipStream_t stream) {
+  const uint64_t blocks_l = std::min<uint64_t>((np + 256) / 256, 4096);
+  hipLaunchKernelGGL(partner_lens, dim3(static_cast<unsigned>(blocks_l)),
+                     dim3(256), 0, stream, partners, np, lens);
+  size_t tb = scan_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, tb, lens, item_off, np + 1,
+                                       stream) != hipSuccess)
+    return -1;
+  if (np > 0) {
+    const uint64_t batches = (np + 63) / 64;
+    const uint64_t blocks = std::min<uint64_t>(
+        (batches + kLayoutWaves - 1) / kLayoutWaves,
+        static_cast<uint64_t>(cus) * 8);
+    hipLaunchKernelGGL(layout_lists, dim3(static_cast<unsigned>(blocks)),
+                       dim3(kLayoutWaves * 64), 0, stream, col, partners, np,
+                       item_off, items);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
@@ -252,8 +374,11 @@ int tc_device_launch(const TcDevicePlan &p, const TcLaunch &l,
   if (l.reset)
     hipLaunchKernelGGL(tc_reset, dim3(1), dim3(64), 0, stream, p.queues,
                        kTcNumClasses, p.result);
-  return l.items == 8 ? launch_all<8>(p, l, stream)
-                      : launch_all<4>(p, l, stream);
+  if (p.items && p.item_off)
+    return l.items == 8 ? launch_all<8, true>(p, l, stream)
+                        : launch_all<4, true>(p, l, stream);
+  return l.items == 8 ? launch_all<8, false>(p, l, stream)
+                      : launch_all<4, false>(p, l, stream);
 }
 
 }  // namespace mgpm

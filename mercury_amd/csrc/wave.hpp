@@ -79,6 +79,15 @@ __device__ __forceinline__ uint32_t wave_first(uint32_t x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
 
+// The same for a 64-bit value, as two 32-bit halves.  The builtin returns
+// int: each half goes back to uint32_t before widening, so a low half of
+// 2^31 or more is not sign-extended into the high half.
+__device__ __forceinline__ uint64_t wave_first_u64(uint64_t x) {
+  const uint32_t lo = wave_first(static_cast<uint32_t>(x));
+  const uint32_t hi = wave_first(static_cast<uint32_t>(x >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // Multiplicative (Fibonacci) hash of a vertex id into 2^logp slots.
 __device__ __forceinline__ uint32_t slot_of(int32_t key, int logp) {
   return (static_cast<uint32_t>(key) * 0x9E3779B1u) >> (32 - logp);

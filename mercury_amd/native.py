@@ -30,7 +30,8 @@ class Tune(C.Structure):
                 ("grid_blocks", C.c_int32), ("pattern_waves", C.c_int32),
                 ("group_work", C.c_int32), ("tc_items", C.c_int32),
                 ("tc_cursor_min", C.c_int32), ("pattern_split", C.c_int32),
-                ("pattern_census", C.c_int32)]
+                ("pattern_census", C.c_int32), ("tc_layout", C.c_int32),
+                ("tc_merge_max", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -1,5 +1,7 @@
 """Triangle-count parity on the GPU: HIP kernels (through the C ABI) versus
-the CPU oracle, bit-exact uint64."""
+the CPU oracle, bit-exact uint64.  Every kernel class runs both with the
+partner lists gathered from the DAG (tc_layout = -1) and streamed from
+their task-by-task layout (tc_layout = 1)."""
 import numpy as np
 import pytest
 
@@ -26,11 +28,16 @@ FORCE = {
     "items8": dict(tc_items=8),
     "items8_global": dict(tc_items=8, small_max_deg=-1, wave_max_deg=-1,
                           block_max_deg=-1),
+    "merge16": dict(tc_merge_max=16),
+    "merge64": dict(tc_merge_max=64),
+    "merge64_items4": dict(tc_merge_max=64, tc_items=4),
 }
 
 
-def _tune(name):
-    kw = FORCE[name]
+def _tune(name, layout=0):
+    kw = dict(FORCE[name] or {})
+    if layout:
+        kw["tc_layout"] = layout
     return ma.make_tune(**kw) if kw else None
 
 
@@ -44,14 +51,39 @@ def _gpu_tc(g, tune=None):
 @pytest.mark.parametrize("scale,ef,seed", [(8, 4, 1), (10, 16, 7),
                                            (12, 8, 3)])
 @pytest.mark.parametrize("mode", list(FORCE))
-def test_tc_rmat_small(gpu_available, scale, ef, seed, mode):
+@pytest.mark.parametrize("layout", [-1, 1])
+def test_tc_rmat_small(gpu_available, scale, ef, seed, mode, layout):
     g = ma.Graph.rmat(scale, ef, seed=seed)
     rp, ci = g.csr()
     want = oracle.tc(rp, ci)
     dg = ma.DeviceGraph(g, 0)
-    got, st = dg.tc_count(_tune(mode))
-    print(f"rmat{scale} ef{ef} seed{seed} {mode}: {got} vs {want}", flush=True)
+    got, st = dg.tc_count(_tune(mode, layout))
+    print(f"rmat{scale} ef{ef} seed{seed} {mode} layout {layout}: {got} vs "
+          f"{want}", flush=True)
     assert got == want
+    assert st.path == (2 if layout > 0 else 1)
+
+
+def test_tc_layout_kept_and_rebuilt(gpu_available):
+    """The laid-out lists are built once and reused; a re-plan (new task
+    cut) drops them and the next streamed count rebuilds them; the gather
+    path still runs beside them; the device-built plan streams too."""
+    g = ma.Graph.rmat(13, 16, seed=5)
+    rp, ci = g.csr()
+    want = oracle.tc(rp, ci)
+    dg = ma.DeviceGraph(g, 0)
+    lay = ma.make_tune(tc_layout=1)
+    c1, s1 = dg.tc_count(lay)
+    c2, s2 = dg.tc_count(lay)
+    assert c1 == c2 == want and s2.prep_ms <= s1.prep_ms
+    assert dg.tc_count(ma.make_tune(tc_layout=-1))[0] == want
+    c3, s3 = dg.tc_count(ma.make_tune(tc_layout=1, chunk_work=9))
+    assert c3 == want and s3.path == 2
+    dg.close()
+    dg = ma.DeviceGraph(g, 0, device_plan=True)
+    for t in (lay, ma.make_tune(tc_layout=-1), lay):
+        assert dg.tc_count(t)[0] == want
+    dg.close()
 
 
 def test_tc_rmat16_seed1(gpu_available):
@@ -67,7 +99,12 @@ def test_tc_rmat16_seed1(gpu_available):
 def test_tc_er(gpu_available):
     g = ma.Graph.er(1 << 14, (1 << 14) * 16, seed=3)
     rp, ci = g.csr()
-    assert _gpu_tc(g)[0] == oracle.tc(rp, ci)
+    want = oracle.tc(rp, ci)
+    assert _gpu_tc(g)[0] == want
+    # every owner small: all tasks merged, from both list sources
+    for layout in (-1, 1):
+        assert _gpu_tc(g, ma.make_tune(tc_merge_max=64,
+                                       tc_layout=layout))[0] == want
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 40])
@@ -94,8 +131,9 @@ def test_tc_dense_hub_uses_big_tables(gpu_available):
                                 for b in range(a + 1, n)])
     g = ma.Graph.from_csr(rp, ci)
     want = n * (n - 1) * (n - 2) // 6
-    for mode in ("default", "global_only", "tiny_chunks"):
-        assert _gpu_tc(g, _tune(mode))[0] == want
+    for mode in ("default", "global_only", "tiny_chunks", "merge64"):
+        for layout in (-1, 1):
+            assert _gpu_tc(g, _tune(mode, layout))[0] == want, (mode, layout)
 
 
 def test_tc_repeatable(gpu_available):
@@ -115,6 +153,8 @@ def test_tc_long_lists_owner_cursor(gpu_available, items):
     want = oracle.tc(rp, ci)
     for mode in ("default", "small_only", "medium_only", "block_only",
                  "global_only", "tiny_chunks"):
-        kw = dict(FORCE[mode] or {})
-        kw["tc_items"] = items
-        assert _gpu_tc(g, ma.make_tune(**kw))[0] == want, mode
+        for layout in (-1, 1):
+            kw = dict(FORCE[mode] or {})
+            kw["tc_items"] = items
+            kw["tc_layout"] = layout
+            assert _gpu_tc(g, ma.make_tune(**kw))[0] == want, (mode, layout)
