@@ -566,6 +566,9 @@ def main() -> int:
                     help="tc mode: 1 = stream laid-out partner lists, -1 = "
                          "gather them from the DAG, 0 = the library's "
                          "default (lay out when the copy fits)")
+    ap.add_argument("--tc-merge-max", type=int, default=0,
+                    help="tc mode: owners with d+ up to this (at most 64) "
+                         "merge partner lists instead of hashing")
     ap.add_argument("--no-device-plan", action="store_true",
                     help="skip the host-plan vs device-plan upload timing")
     ap.add_argument("--mode", default="tc",
@@ -643,7 +646,9 @@ def main() -> int:
     t0 = time.perf_counter()
     dg = ma.DeviceGraph(g, 0)
     upload_s = time.perf_counter() - t0
-    tune = ma.make_tune(tc_layout=args.tc_layout) if args.tc_layout else None
+    kw = {k: v for k, v in (("tc_layout", args.tc_layout),
+                            ("tc_merge_max", args.tc_merge_max)) if v}
+    tune = ma.make_tune(**kw) if kw else None
     # plan preparation outside the warm-up and timed steps: the first count
     # lays the partner lists out (when it does), reported as prep
     prep_ms = dg.tc_count(tune)[1].prep_ms

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int cursor_min, int merge_max) {
   __shared__ int32_t s_table[WAVES][1 << LOGP_MAX];
-  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
+  __shared__ int32_t s_mark[WAVES][STREAM ? 1 : 64 * ITEMS];  // start marks
   const int lane = lane64();
   const int wv = threadIdx.x >> 6;
   int32_t *table = s_table[wv];
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_block_tables(
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int logp_max, int cursor_min) {
   extern __shared__ int32_t s_table[];
-  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
+  __shared__ int32_t s_mark[WAVES][STREAM ? 1 : 64 * ITEMS];  // start marks
   __shared__ uint32_t s_task;
   const int wv = threadIdx.x >> 6;
   unsigned long long acc = 0;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
     const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
     unsigned *__restrict__ queue, unsigned long long *__restrict__ result,
     int cursor_min) {
-  __shared__ int32_t s_mark[WAVES][64 * ITEMS];
+  __shared__ int32_t s_mark[WAVES][STREAM ? 1 : 64 * ITEMS];  // start marks
   const int lane = lane64();
   const int wv = threadIdx.x >> 6;
   unsigned long long acc = 0;
