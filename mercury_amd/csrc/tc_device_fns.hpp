@@ -455,14 +455,37 @@ __device__ __forceinline__ uint32_t wave_task_merge(
   return hits;
 }
 
+// The first step of a task's laid-out items (64 * ITEMS from ib), loaded
+// ahead of the task: every load is issued, at an address clamped into the
+// task's items (a task has at least one), and the flags say which count.
+template <int ITEMS>
+struct FirstStep {
+  int32_t v[ITEMS];
+};
+
+template <int ITEMS>
+__device__ __forceinline__ FirstStep<ITEMS> load_first_step(
+    const int32_t *__restrict__ items, uint64_t ib, uint64_t ie, int lane) {
+  FirstStep<ITEMS> f;
+  const uint64_t last = ie - 1;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint64_t idx = ib + uint64_t(64 * i + lane);
+    f.v[i] = items[idx < last ? idx : last];
+  }
+  return f;
+}
+
 // wave_task over the task's laid-out items (pre.ib .. pre.ie) instead of
-// its partner lists.
+// its partner lists; `first` (if non-null) holds its first step, loaded
+// ahead by load_first_step.
 template <int ITEMS>
 __device__ __forceinline__ uint32_t wave_task_stream(
     const int32_t *__restrict__ col, const int32_t *__restrict__ items,
-    const TcTask &task, int32_t *table, int logp_max, const TaskPre &pre) {
+    const TcTask &task, int32_t *table, int logp_max, const TaskPre &pre,
+    const FirstStep<ITEMS> *first = nullptr) {
   const int lane = lane64();
-  const uint64_t ib = wave_first_u64(pre.ib);
+  uint64_t ib = wave_first_u64(pre.ib);
   const uint64_t ie = wave_first_u64(pre.ie);
   if (2 * task.tlen > (1 << logp_max)) {
     SortedProbe probe{col + task.tbeg, task.tlen};
@@ -480,13 +503,32 @@ __device__ __forceinline__ uint32_t wave_task_stream(
     table_insert(table, logp, col, task.tbeg, task.tlen, lane, 64,
                  logp_max > 8);
   wave_sync();
-  uint32_t hits;
+  uint32_t hits = 0;
+  int32_t fv[ITEMS];
+  bool fok[ITEMS];
+  if (first) {  // the first step from registers, the rest streamed
+    const int left = ie - ib < uint64_t(64 * ITEMS)
+                         ? static_cast<int>(ie - ib) : 64 * ITEMS;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      fok[i] = 64 * i + lane < left;
+      fv[i] = fok[i] ? first->v[i] : -2;
+    }
+  }
   if (logp_max <= 8) {
     HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
-    hits = stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
+    if (first) {
+      hits = probe.template lookup<ITEMS>(fv, fok);
+      ib += 64 * ITEMS;
+    }
+    hits += stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
   } else {
     HashProbe<true> probe{table, logp, static_cast<uint32_t>(P - 1)};
-    hits = stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
+    if (first) {
+      hits = probe.template lookup<ITEMS>(fv, fok);
+      ib += 64 * ITEMS;
+    }
+    hits += stream_items<ITEMS>(items, ib, ie, probe, 0, 1);
   }
   wave_sync();
   return hits;

@@ -56,6 +56,41 @@ __global__ __launch_bounds__(WAVES * 64) void tc_wave_tables(
     if (g >= static_cast<uint32_t>(ngroups)) break;
     const int t_end = goff[g + 1];
     int t = goff[g];
+    if constexpr (STREAM) {
+      // A three-stage pipeline over the group's tasks, so that a task's
+      // loads are all issued in earlier iterations: iteration t issues the
+      // record of task t + 3, the item offsets and owner items of t + 2
+      // (from its record, loaded one iteration earlier) and the first step
+      // of items of t + 1 (from its offsets, loaded one iteration earlier).
+      // Indices past the group are clamped to its last task.
+      const int last = t_end - 1;
+      TcTask r0 = tasks[t];
+      TcTask r1 = tasks[t + 1 < last ? t + 1 : last];
+      TcTask r2 = tasks[t + 2 < last ? t + 2 : last];
+      TaskPre p0 = load_task_pre<true>(col, partners, r0, lane, item_off);
+      TaskPre p1 = load_task_pre<true>(col, partners, r1, lane, item_off);
+      FirstStep<ITEMS> f0 = load_first_step<ITEMS>(items, p0.ib, p0.ie, lane);
+      for (; t < t_end; ++t) {
+        const TcTask r3 = tasks[t + 3 < last ? t + 3 : last];
+        const TaskPre p2 =
+            load_task_pre<true>(col, partners, r2, lane, item_off);
+        const FirstStep<ITEMS> f1 =
+            load_first_step<ITEMS>(items, p1.ib, p1.ie, lane);
+        if (r0.tlen <= merge_max)
+          acc += wave_task_merge(col, partners, items, item_off, r0, table,
+                                 p0);
+        else
+          acc += wave_task_stream<ITEMS>(col, items, r0, table, LOGP_MAX, p0,
+                                         &f0);
+        r0 = r1;
+        r1 = r2;
+        r2 = r3;
+        p0 = p1;
+        p1 = p2;
+        f0 = f1;
+      }
+      continue;
+    }
     TcTask task = tasks[t];
     TaskPre pre = load_task_pre<STREAM>(col, partners, task, lane, item_off);
     for (; t < t_end; ++t) {

@@ -103,22 +103,33 @@ def test_pairs_closed_forms(gpu_available):
     assert counts(8, []) == {2: 0, 3: 0, 5: 0, 6: 0}
 
 
+_GNP = {}
+
+
+def _gnp(n, pr, seed):
+    if (n, pr, seed) not in _GNP:
+        rng = random.Random(seed)
+        edges = [(a, b) for a in range(n) for b in range(a + 1, n)
+                 if rng.random() < pr]
+        _GNP[(n, pr, seed)] = csr_from_edges(n, edges)
+    return _GNP[(n, pr, seed)]
+
+
 @pytest.mark.parametrize("n,pr,seed", [(300, 0.45, 1), (700, 0.45, 2),
                                        (1400, 0.42, 3)])
-def test_pairs_every_anchor_class_vs_kernels(gpu_available, n, pr, seed):
+@pytest.mark.parametrize("p", PATTERNS)
+def test_pairs_every_anchor_class_vs_kernels(gpu_available, n, pr, seed, p):
     """Dense G(n, p): DAG out-degrees reach past 128, 256 and 512, so every
     anchor class (LDS rows of 1..8 words, and the global-row class) runs;
-    the per-edge kernels give the reference's count."""
-    rng = random.Random(seed)
-    edges = [(a, b) for a in range(n) for b in range(a + 1, n)
-             if rng.random() < pr]
-    rp, ci = csr_from_edges(n, edges)
+    the per-edge kernels give the reference's count (one pattern per test:
+    the per-edge kernels take up to a minute on n = 1400)."""
+    rp, ci = _gnp(n, pr, seed)
     dg = ma.DeviceGraph(ma.Graph.from_csr(rp, ci), 0)
-    for p in PATTERNS:
-        a = _census(dg, p)
-        b = dg.pattern_count(p, ma.make_tune(**SET_ALGEBRA))[0]
-        print(f"P{p} gnp n={n} p={pr}: census {a} kernels {b}", flush=True)
-        assert a == b, p
+    a = _census(dg, p)
+    b = dg.pattern_count(p, ma.make_tune(**SET_ALGEBRA))[0]
+    print(f"P{p} gnp n={n} p={pr}: census {a} kernels {b}", flush=True)
+    assert a == b, p
+    dg.close()
 
 
 @pytest.mark.parametrize("scale,seed", [(12, 4), (13, 1)])
