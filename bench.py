@@ -232,11 +232,8 @@ def run_batch(ma, args) -> dict:
     if args.batch_warmup > 0:
         for _ in range(args.batch_warmup):
             counts, st = b.run(queries)
-    else:
-        # the lanes' workspaces grown once, by the queries of the largest
-        # graphs (one graph per lane)
-        big = sorted(range(len(dgs)), key=lambda k: -int(graphs[k].nnz))
-        b.run([(dgs[k], p) for k in big[:args.batch_lanes] for p in pats])
+    # (without warm-up batches the lanes' workspaces grow inside the timed
+    # batch: one allocation per lane, milliseconds against its seconds)
     sync()
     log("batch: warm-up batch done")
     t_batch, kms = [], []
@@ -253,7 +250,7 @@ def run_batch(ma, args) -> dict:
     # the oracle on the small graphs (its pattern algebra is cubic)
     checked, oracle_ok = 0, True
     for k, ((s_, ef), g) in enumerate(zip(specs, graphs)):
-        if s_ > 10 or ef > 16:
+        if s_ > 10 or ef > args.batch_oracle_ef:
             continue
         rp, ci = g.csr()
         for j, p in enumerate(pats):
@@ -516,6 +513,7 @@ def run_extras(ma, args) -> dict:
     if "batch" in extras:
         a = copy.copy(args)
         a.batch_warmup = 0
+        a.batch_oracle_ef = min(args.batch_oracle_ef, 8)
         out["batch"] = _guarded("batch", run_batch, ma, a)
     if "serve" in extras:
         def serve():
@@ -598,16 +596,19 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-warmup", type=int, default=1)
     ap.add_argument("--batch-patterns", default="TC,P2,P4")
+    ap.add_argument("--batch-oracle-ef", type=int, default=16,
+                    help="batch: the oracle checks the graphs of scale <= 10 "
+                         "and edge factor <= this")
     ap.add_argument("--extras", default="cpu,batch,serve",
                     help="tc mode: sub-objects added to the line (cpu: the "
                          "small CPU baselines; batch: the batch workload; "
                          "serve: the served stream); '' for none")
-    ap.add_argument("--extras-serve-requests", type=int, default=80)
+    ap.add_argument("--extras-serve-requests", type=int, default=60)
     ap.add_argument("--extras-alone-checks", type=int, default=20)
     ap.add_argument("--serve-low-rate", type=float, default=0.0,
                     help="below-saturation rate (0: 0.4 x the batched "
                          "throughput at the offered rate)")
-    ap.add_argument("--serve-low-requests", type=int, default=20)
+    ap.add_argument("--serve-low-requests", type=int, default=12)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device-to-device copy of MIB MiB "
                          "(5 times) to calibrate rocprofv3 FETCH_SIZE / "

@@ -48,6 +48,61 @@ __device__ __forceinline__ bool contains_g(const int32_t *__restrict__ list,
   return i < len && list[i] == key;
 }
 
+// How many of K keys (flags ok) lie in the sorted list[0, len), len >= 1:
+// K branch-free searches for the last element <= key run in lockstep (the
+// window shrinks by the same half for every key), so their loads overlap.
+// For K keys, the position of the last element <= key in the sorted
+// list[0, len), len >= 1 (0 if every element is larger): the searches of
+// found_in, returning positions.
+template <int K>
+__device__ __forceinline__ void last_le(const int32_t *__restrict__ list,
+                                        int len, const int32_t (&key)[K],
+                                        int (&b)[K]) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) b[j] = 0;
+  for (int n = len; n > 1;) {
+    const int half = n >> 1;
+    int32_t v[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = list[b[j] + half];
+#pragma unroll
+    for (int j = 0; j < K; ++j) b[j] = v[j] <= key[j] ? b[j] + half : b[j];
+    n -= half;
+  }
+}
+
+template <int K>
+__device__ __forceinline__ void found_in(const int32_t *__restrict__ list,
+                                         int len, const int32_t (&key)[K],
+                                         const bool (&ok)[K], bool (&found)[K]) {
+  int b[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) b[j] = 0;
+  for (int n = len; n > 1;) {
+    const int half = n >> 1;
+    int32_t v[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = list[b[j] + half];
+#pragma unroll
+    for (int j = 0; j < K; ++j) b[j] = v[j] <= key[j] ? b[j] + half : b[j];
+    n -= half;
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) found[j] = ok[j] && list[b[j]] == key[j];
+}
+
+template <int K>
+__device__ __forceinline__ int count_in(const int32_t *__restrict__ list,
+                                        int len, const int32_t (&key)[K],
+                                        const bool (&ok)[K]) {
+  bool found[K];
+  found_in<K>(list, len, key, ok, found);
+  int hits = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) hits += found[j] ? 1 : 0;
+  return hits;
+}
+
 __global__ void dag_sources(const int64_t *__restrict__ drp, int32_t n,
                             int32_t *__restrict__ dsrc) {
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
@@ -77,15 +132,24 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
     const int32_t *S = c.dcol + sb;
     const int32_t *L = c.dcol + lb;
     int cnt = 0;
-    for (int p0 = 0; p0 < ls; p0 += 64) {
-      const int p = p0 + lane;
-      bool hit = false;
-      int32_t z = 0;
-      if (p < ls) {
-        z = S[p];
-        const int j = lower_bound_g(L, ll, z);
-        if (j < ll && L[j] == z) {
-          hit = true;
+    // 4 items of S per lane at a time, searched in L in lockstep; hits are
+    // compacted in item order, so C stays sorted.
+    for (int p0 = 0; ls > 0 && ll > 0 && p0 < ls; p0 += 4 * 64) {
+      int32_t zk[4];
+      int jk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + 64 * k + lane;
+        zk[k] = S[p < ls ? p : ls - 1];
+      }
+      last_le<4>(L, ll, zk, jk);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + 64 * k + lane;
+        const int32_t z = zk[k];
+        const int j = jk[k];
+        const bool hit = p < ls && L[j] == z;
+        if (hit) {
           atomicAdd(&c.t[sb + p], 1u);
           atomicAdd(&c.t[lb + j], 1u);
           const unsigned long long dz = static_cast<unsigned long long>(c.deg[z]);
@@ -98,10 +162,10 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
             bdeg += ex * ey + ex * ez + ey * ez;
           }
         }
+        const unsigned long long mask = __ballot(hit);
+        if (hit) slab[cnt + bits_below(mask)] = z;
+        cnt += __popcll(mask);
       }
-      const unsigned long long mask = __ballot(hit);
-      if (hit) slab[cnt + bits_below(mask)] = z;
-      cnt += __popcll(mask);
     }
     if (cnt == 0) continue;
     if (lane == 0) {
@@ -258,26 +322,32 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
     const int32_t *S = c.dcol + sb;
     const int32_t *Lg = c.dcol + lb;
     int cnt = 0;
-    for (int p0 = 0; p0 < ls; p0 += 64) {
-      const int p = p0 + lane;
-      bool hit = false;
-      int32_t z = 0;
-      int j = 0;
-      if (p < ls) {
-        z = S[p];
-        j = lower_bound_g(Lg, ll, z);
-        hit = j < ll && Lg[j] == z;
+    for (int p0 = 0; ls > 0 && ll > 0 && p0 < ls; p0 += 4 * 64) {
+      int32_t zk[4];
+      int jk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + 64 * k + lane;
+        zk[k] = S[p < ls ? p : ls - 1];
       }
-      const unsigned long long mask = __ballot(hit);
-      if (hit) {
-        const int o = cnt + bits_below(mask);
-        const uint32_t is = static_cast<uint32_t>(sb + p);
-        const uint32_t il = static_cast<uint32_t>(lb + j);
-        Cz[o] = z;
-        Cxz[o] = xs ? is : il;
-        Cyz[o] = xs ? il : is;
+      last_le<4>(Lg, ll, zk, jk);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + 64 * k + lane;
+        const int32_t z = zk[k];
+        const int j = jk[k];
+        const bool hit = p < ls && Lg[j] == z;
+        const unsigned long long mask = __ballot(hit);
+        if (hit) {
+          const int o = cnt + bits_below(mask);
+          const uint32_t is = static_cast<uint32_t>(sb + p);
+          const uint32_t il = static_cast<uint32_t>(lb + j);
+          Cz[o] = z;
+          Cxz[o] = xs ? is : il;
+          Cyz[o] = xs ? il : is;
+        }
+        cnt += __popcll(mask);
       }
-      cnt += __popcll(mask);
     }
     if (cnt == 0) continue;  // x and y are not the lowest of any triangle
     // F = N(x) ∩ N(y) over the undirected lists.
@@ -368,11 +438,23 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
         const int32_t z = Cz[i];
         const int dz = c.deg[z];
         const int32_t *Nz = c.col + c.rowptr[z];
+        // The shorter of F and N(z) searched in the other, 4 keys per lane
+        // at a time (idle keys repeat a valid one and are not counted).
+        const bool fz = te <= dz;
+        const int32_t *keys = fz ? F : Nz;
+        const int32_t *list = fz ? Nz : F;
+        const int nk = fz ? te : dz, nl = fz ? dz : te;
         int part = 0;
-        if (te <= dz) {
-          for (int q = lane; q < te; q += 64) part += contains_g(Nz, dz, F[q]);
-        } else {
-          for (int q = lane; q < dz; q += 64) part += contains_g(F, te, Nz[q]);
+        for (int q0 = 0; q0 < nk; q0 += 4 * 64) {
+          int32_t key[4];
+          bool ok[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int q = q0 + 64 * j + lane;
+            ok[j] = q < nk;
+            key[j] = keys[ok[j] ? q : nk - 1];
+          }
+          part += count_in<4>(list, nl, key, ok);
         }
         const unsigned k = static_cast<unsigned>(wave_sum_i32(part));
         if (lane == 0 && k) {
@@ -410,11 +492,23 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
           const int64_t bi = c.drp[zi];
           const int di = static_cast<int>(c.drp[zi + 1] - bi);
           const int32_t *Ni = c.dcol + bi;
-          for (int w = 0; w < R; ++w) {
-            const int j = w * 64 + lane;
-            const bool adj = j < cnt && contains_g(Ni, di, Cz[j]);
-            const unsigned long long mask = __ballot(adj);
-            if (lane == 0) rows[int64_t(i) * R + w] = mask;
+          for (int w0 = 0; w0 < R; w0 += 4) {  // 4 row words per search
+            int32_t key[4];
+            bool ok[4], adj[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int j = (w0 + k) * 64 + lane;
+              ok[k] = w0 + k < R && j < cnt;
+              key[k] = Cz[ok[k] ? j : 0];
+              adj[k] = false;
+            }
+            if (di > 0) found_in<4>(Ni, di, key, ok, adj);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              if (w0 + k >= R) break;  // uniform
+              const unsigned long long mask = __ballot(adj[k]);
+              if (lane == 0) rows[int64_t(i) * R + w0 + k] = mask;
+            }
           }
         }
       }
