@@ -91,6 +91,40 @@ __device__ __forceinline__ void found_in(const int32_t *__restrict__ list,
   for (int j = 0; j < K; ++j) found[j] = ok[j] && list[b[j]] == key[j];
 }
 
+// found_in with a list per key (lengths >= 1): each key's window halves
+// until it is one element; the loop runs as long as the longest list
+// needs, keys whose window is done re-read their element.
+template <int K>
+__device__ __forceinline__ void found_in_each(const int32_t *const (&list)[K],
+                                              const int (&len)[K],
+                                              const int32_t (&key)[K],
+                                              const bool (&ok)[K],
+                                              bool (&found)[K]) {
+  int b[K], n[K], nmax = 1;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    b[j] = 0;
+    n[j] = len[j];
+    nmax = n[j] > nmax ? n[j] : nmax;
+  }
+  for (; nmax > 1; nmax -= nmax >> 1) {
+    int32_t v[K];
+    int half[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      half[j] = n[j] >> 1;
+      v[j] = list[j][b[j] + half[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      b[j] = v[j] <= key[j] ? b[j] + half[j] : b[j];
+      n[j] -= half[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < K; ++j) found[j] = ok[j] && list[j][b[j]] == key[j];
+}
+
 template <int K>
 __device__ __forceinline__ int count_in(const int32_t *__restrict__ list,
                                         int len, const int32_t (&key)[K],
@@ -177,16 +211,32 @@ __global__ __launch_bounds__(kWaves * 64) void census_triangles(CensusArgs c) {
     if (cnt < 2) continue;
     wave_sync();  // the slab entries written by other lanes
     if (cnt <= kPairMax) {
+      // Ordered pairs (i, j) of C, 4 per lane at a time: is C[j] in N+(C[i])?
       const int np = cnt * cnt;
-      for (int p0 = 0; p0 < np; p0 += 64) {
-        const int p = p0 + lane;
-        if (p >= np) continue;
-        const int i = p / cnt, j = p - i * cnt;
-        if (i == j) continue;
-        const int32_t z = slab[i], q = slab[j];
-        const int64_t bz = c.drp[z];
-        if (contains_g(c.dcol + bz, static_cast<int>(c.drp[z + 1] - bz), q))
-          ++k4;
+      for (int p0 = 0; p0 < np; p0 += 4 * 64) {
+        const int32_t *lst[4];
+        int len[4];
+        int32_t key[4];
+        bool ok[4], hit[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int p = p0 + 64 * k + lane;
+          const int pp = p < np ? p : 0;
+          const int i = pp / cnt, j = pp - i * cnt;
+          const int32_t z = slab[i];
+          const int64_t bz = c.drp[z];
+          len[k] = static_cast<int>(c.drp[z + 1] - bz);
+          ok[k] = p < np && i != j && len[k] > 0;
+          lst[k] = c.dcol + bz;
+          if (len[k] < 1) {  // a one-element window at a valid address
+            lst[k] = slab;
+            len[k] = 1;
+          }
+          key[k] = slab[j];
+        }
+        found_in_each<4>(lst, len, key, ok, hit);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) k4 += hit[k] ? 1 : 0;
       }
     } else {
       for (int i = 0; i < cnt; ++i) {
@@ -357,17 +407,22 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
     const int32_t *Sr = c.col + c.rowptr[wx ? y : x];
     const int nw = wx ? dx : dy, ns = wx ? dy : dx;
     int te = 0;
-    for (int p0 = 0; p0 < nw; p0 += 64) {
-      const int p = p0 + lane;
-      bool hit = false;
-      int32_t w = 0;
-      if (p < nw) {
-        w = W[p];
-        hit = contains_g(Sr, ns, w);
+    for (int p0 = 0; nw > 0 && p0 < nw; p0 += 4 * 64) {  // 4 per lane, in order
+      int32_t wk[4];
+      bool ok[4], hk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int p = p0 + 64 * k + lane;
+        ok[k] = p < nw;
+        wk[k] = W[ok[k] ? p : nw - 1];
       }
-      const unsigned long long mask = __ballot(hit);
-      if (hit) F[te + bits_below(mask)] = w;
-      te += __popcll(mask);
+      found_in<4>(Sr, ns, wk, ok, hk);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const unsigned long long mask = __ballot(hk[k]);
+        if (hk[k]) F[te + bits_below(mask)] = wk[k];
+        te += __popcll(mask);
+      }
     }
     if (lane == 0 && static_cast<uint32_t>(te) != ta) ++bad;
     wave_sync();  // C and F written by other lanes
@@ -406,21 +461,37 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
         s_base[wv][lane] = bz;
         s_hits[wv][lane] = 0;
         wave_sync();  // the per-triangle LDS entries
-        for (int p = lane; p < total; p += 64) {
-          // r = the last triangle whose run starts at or before p.
-          int lo = 0, hi = 64;
-          while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (s_off[wv][mid] <= p) lo = mid + 1;
-            else hi = mid;
+        for (int p0 = 0; p0 < total; p0 += 2 * 64) {  // 2 tests per lane
+          const int32_t *lst[2];
+          int len[2], rk[2];
+          int32_t key[2];
+          bool ok[2], hit[2];
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const int p = p0 + 64 * k + lane;
+            ok[k] = p < total;
+            const int pp = ok[k] ? p : 0;
+            // r = the last triangle whose run starts at or before pp.
+            int lo = 0, hi = 64;
+            while (lo < hi) {
+              const int mid = (lo + hi) >> 1;
+              if (s_off[wv][mid] <= pp) lo = mid + 1;
+              else hi = mid;
+            }
+            const int r = lo - 1;
+            const int q = pp - s_off[wv][r];
+            const int dr = s_dz[wv][r];
+            const int32_t *Nz = c.col + s_base[wv][r];
+            const bool fz = te <= dr;  // search F[q] in N(z), else Nz[q] in F
+            lst[k] = fz ? Nz : F;
+            len[k] = fz ? dr : te;
+            key[k] = fz ? F[q] : Nz[q];
+            rk[k] = r;
           }
-          const int r = lo - 1;
-          const int q = p - s_off[wv][r];
-          const int dr = s_dz[wv][r];
-          const int32_t *Nz = c.col + s_base[wv][r];
-          const bool hit = te <= dr ? contains_g(Nz, dr, F[q])
-                                    : contains_g(F, te, Nz[q]);
-          if (hit) atomicAdd(&s_hits[wv][r], 1u);
+          found_in_each<2>(lst, len, key, ok, hit);
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            if (hit[k]) atomicAdd(&s_hits[wv][rk[k]], 1u);
         }
         wave_sync();  // the hit counts
         const unsigned k = s_hits[wv][lane];
