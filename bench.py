@@ -567,6 +567,9 @@ def main() -> int:
     ap.add_argument("--tc-merge-max", type=int, default=0,
                     help="tc mode: owners with d+ up to this (at most 64) "
                          "merge partner lists instead of hashing")
+    ap.add_argument("--tc-items", type=int, default=0,
+                    help="tc mode: list items per lane per step (4 or 8; 0 = "
+                         "the library's default)")
     ap.add_argument("--no-device-plan", action="store_true",
                     help="skip the host-plan vs device-plan upload timing")
     ap.add_argument("--mode", default="tc",
@@ -648,7 +651,8 @@ def main() -> int:
     dg = ma.DeviceGraph(g, 0)
     upload_s = time.perf_counter() - t0
     kw = {k: v for k, v in (("tc_layout", args.tc_layout),
-                            ("tc_merge_max", args.tc_merge_max)) if v}
+                            ("tc_merge_max", args.tc_merge_max),
+                            ("tc_items", args.tc_items)) if v}
     tune = ma.make_tune(**kw) if kw else None
     # plan preparation outside the warm-up and timed steps: the first count
     # lays the partner lists out (when it does), reported as prep
