@@ -567,6 +567,9 @@ def main() -> int:
     ap.add_argument("--tc-merge-max", type=int, default=0,
                     help="tc mode: owners with d+ up to this (at most 64) "
                          "merge partner lists instead of hashing")
+    ap.add_argument("--tc-half-waves", type=int, default=0,
+                    help="tc mode: 1 = small-class tasks two per wave, -1 = "
+                         "one per wave, 0 = the library's default")
     ap.add_argument("--tc-items", type=int, default=0,
                     help="tc mode: list items per lane per step (4 or 8; 0 = "
                          "the library's default)")
@@ -652,7 +655,8 @@ def main() -> int:
     upload_s = time.perf_counter() - t0
     kw = {k: v for k, v in (("tc_layout", args.tc_layout),
                             ("tc_merge_max", args.tc_merge_max),
-                            ("tc_items", args.tc_items)) if v}
+                            ("tc_items", args.tc_items),
+                            ("tc_half_waves", args.tc_half_waves)) if v}
     tune = ma.make_tune(**kw) if kw else None
     # plan preparation outside the warm-up and timed steps: the first count
     # lays the partner lists out (when it does), reported as prep

@@ -114,6 +114,10 @@ typedef struct mgpm_tune {
                              skip the hash table: each lane merges one
                              partner list with the owner's sorted list (two
                              cursors); 0 (default) = never, -1 = never */
+  int32_t tc_half_waves;  /* TC with laid-out lists: 1 = small-class tasks
+                             (d+ <= 128) two per wave, 32 lanes each with
+                             its own table; -1 = one per wave; 0 (default)
+                             = two per wave when every task is small */
 } mgpm_tune;
 
 /* Per-call measurements. */

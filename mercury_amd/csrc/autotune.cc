@@ -36,6 +36,7 @@ mgpm_tune tune_of(const TcResident &r) {
   t.tc_cursor_min = r.cursor_min;
   t.tc_layout = r.layout;
   t.tc_merge_max = r.merge_max == 0 ? -1 : r.merge_max;
+  t.tc_half_waves = r.half_waves;
   return t;
 }
 
@@ -85,6 +86,7 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
   const std::vector<Knob> knobs = {
       {&mgpm_tune::tc_layout, {-1, 0}},
       {&mgpm_tune::tc_merge_max, {-1, 16, 32, 64}},
+      {&mgpm_tune::tc_half_waves, {-1, 1}},
       {&mgpm_tune::tc_items, {4, 8}},
       {&mgpm_tune::tc_cursor_min, {-1, 16, 32, 64}},
       {&mgpm_tune::small_max_deg, {32, 64, 128}},
@@ -118,6 +120,7 @@ int tune_tc(mgpm_dgraph *dg, int reps, mgpm_tune *best, double *best_ms) {
   dg->tc.cursor_min = cur.tc_cursor_min;
   dg->tc.layout = cur.tc_layout;
   dg->tc.merge_max = cur.tc_merge_max > 0 ? cur.tc_merge_max : 0;
+  dg->tc.half_waves = cur.tc_half_waves;
   dg->tc.tuned = true;
   *best = cur;
   *best_ms = cur_ms;

@@ -577,6 +577,14 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   l.merge_max = (tune && tune->tc_merge_max != 0) ? tune->tc_merge_max
                                                   : dg->tc.merge_max;
   if (l.merge_max < 0) l.merge_max = 0;
+  const int hw = (tune && tune->tc_half_waves != 0) ? tune->tc_half_waves
+                                                    : dg->tc.half_waves;
+  // Default: two small tasks per wave when the plan has only small tasks
+  // (ER-like graphs, where per-task costs dominate).
+  const TcDevicePlan &dp = dg->tc.dev;
+  l.half_small = hw > 0 || (hw == 0 && dp.ngroups[kTcMedium] == 0 &&
+                            dp.ngroups[kTcBlock] == 0 &&
+                            dp.ngroups[kTcGlobal] == 0);
   // Laid-out partner lists: asked for (1), refused (-1), or by default
   // when they fit a quarter of the free device memory.
   const int lay = (tune && tune->tc_layout != 0) ? tune->tc_layout

@@ -84,6 +84,7 @@ struct TcResident {
   int32_t max_out = 0;
   int layout = 0;           // tc_layout choice of the resident plan
   int merge_max = 0;        // tc_merge_max choice (0: never)
+  int half_waves = 0;       // tc_half_waves choice (0: the default)
   DevBuf col, partners, tasks[kTcNumClasses], groups[kTcNumClasses];
   // The plan's partner lists laid out task by task (tc_layout), built on
   // first use and dropped with the plan.

@@ -42,6 +42,8 @@ struct TcLaunch {
                         // length of a 64-partner batch (<= 0: never)
   int merge_max = 0;    // owners of d+ up to this (at most 64) merge their
                         // partner lists lane by lane instead of hashing
+  bool half_small = false;  // laid-out lists: small-class tasks two per
+                            // wave (tc_half_tables)
 };
 
 // Enqueues reset + counting kernels on `stream`; the count lands in

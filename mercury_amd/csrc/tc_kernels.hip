@@ -32,6 +32,8 @@ namespace {
 
 using namespace tcdev;
 
+constexpr int kSmallLogpHalf = 8;  // half-wave tables: 256 slots
+
 // Wave-private tables of up to 2^LOGP_MAX slots (tasks with 2*d+ <= that).
 // STREAM: every task streams its laid-out items (items[item_off[pbeg] ..
 // item_off[pbeg + pcnt])) instead of gathering its partner lists.
@@ -209,6 +211,87 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
   flush_hits(acc, result);
 }
 
+// Small-class tasks on laid-out lists, two per wave: lanes 0-31 take the
+// group's tasks t0, t0 + 2, ... and lanes 32-63 t0 + 1, t0 + 3, ..., each
+// half with its own table (up to 256 slots) and its own items, 32 x ITEMS
+// per step; the halves step together (a half without a task, or with
+// fewer steps, runs idle items).  Tiny tasks (ER-like graphs: about 16
+// owner items and 170 streamed items) then share each wave's per-task
+// costs -- table clear and fill, synchronisation, the load chain.
+template <int WAVES, int ITEMS>
+__global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
+    const int32_t *__restrict__ col, const TcTask *__restrict__ tasks,
+    const int32_t *__restrict__ goff, int ngroups,
+    const int32_t *__restrict__ items, const uint64_t *__restrict__ item_off,
+    unsigned *__restrict__ queue, unsigned long long *__restrict__ result) {
+  __shared__ int32_t s_table[WAVES][2][1 << kSmallLogpHalf];
+  const int lane = lane64();
+  const int wv = threadIdx.x >> 6;
+  const int sub = lane >> 5, gl = lane & 31;
+  int32_t *table = s_table[wv][sub];
+  unsigned long long acc = 0;
+  for (;;) {
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(queue, 1u);
+    g = wave_first(g);
+    if (g >= static_cast<uint32_t>(ngroups)) break;
+    const int t0 = goff[g], t_end = goff[g + 1];
+    // The next pair's record, item offsets and first owner item are loaded
+    // one pair ahead (an idle half reads the other half's task).
+    auto rec_of = [&](int t) {
+      const int tt = t + sub;
+      return tasks[tt < t_end ? tt : (t < t_end ? t : t_end - 1)];
+    };
+    TcTask task = rec_of(t0);
+    uint64_t ib = item_off[task.pbeg];
+    uint64_t ie = item_off[task.pbeg + static_cast<uint32_t>(task.pcnt)];
+    for (int t = t0; t < t_end; t += 2) {
+      const bool have = t + sub < t_end;
+      const TcTask next = rec_of(t + 2);
+      const uint64_t nib = item_off[next.pbeg];
+      const uint64_t nie =
+          item_off[next.pbeg + static_cast<uint32_t>(next.pcnt)];
+      int logp = ceil_log2(8 * task.tlen);
+      logp = logp < 5 ? 5 : (logp > kSmallLogpHalf ? kSmallLogpHalf : logp);
+      const int P = 1 << logp;
+      for (int i = gl; i < P; i += 32) table[i] = -1;
+      wave_sync();
+      if (have) {
+        const uint32_t mask = static_cast<uint32_t>(P - 1);
+        for (int k = gl; k < task.tlen; k += 32) {
+          const int32_t key = col[task.tbeg + static_cast<uint32_t>(k)];
+          uint32_t h = slot_of(key, logp);
+          while (atomicCAS(&table[h], -1, key) != -1) h = (h + 1) & mask;
+        }
+      }
+      wave_sync();
+      HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
+      const uint64_t n = have ? ie - ib : 0;
+      const uint64_t steps = (n + 32 * ITEMS - 1) / (32 * ITEMS);
+      const uint64_t other = __shfl_xor(steps, 32, 64);
+      const uint64_t wsteps = steps > other ? steps : other;
+      for (uint64_t st = 0; st < wsteps; ++st) {
+        int32_t w[ITEMS];
+        bool ok[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+          const uint64_t k = st * (32 * ITEMS) + uint64_t(32 * i + gl);
+          ok[i] = k < n;
+          const uint64_t idx = ib + (ok[i] ? k : 0);  // clamped: ib < ie
+          const int32_t v = items[idx];
+          w[i] = ok[i] ? v : -2;
+        }
+        acc += probe.template lookup<ITEMS>(w, ok);
+      }
+      wave_sync();  // the tables are cleared for the next pair
+      task = next;
+      ib = nib;
+      ie = nie;
+    }
+  }
+  flush_hits(acc, result);
+}
+
 __global__ void tc_reset(unsigned *queues, int nq,
                          unsigned long long *result) {
   const int i = threadIdx.x;
@@ -269,7 +352,17 @@ int launch_all(const TcDevicePlan &p, const TcLaunch &l, hipStream_t stream) {
                        p.item_off, p.queues + kTcMedium, p.result,
                        l.cursor_min, merge_max);
   }
-  if (p.ngroups[kTcSmall] > 0) {
+  if (STREAM && l.half_small && merge_max == 0 && p.ngroups[kTcSmall] > 0 &&
+      p.class_max_tlen[kTcSmall] <= kTcSmallMaxDeg) {
+    auto *kern = &tc_half_tables<kWaves, ITEMS>;
+    const int grid = grid_for(reinterpret_cast<const void *>(kern),
+                              kWaves * 64, 0, l.cus, l.grid_cap,
+                              waves_to_blocks(p.ngroups[kTcSmall], kWaves));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWaves * 64), 0, stream, p.col,
+                       p.tasks[kTcSmall], p.groups[kTcSmall],
+                       p.ngroups[kTcSmall], p.items, p.item_off,
+                       p.queues + kTcSmall, p.result);
+  } else if (p.ngroups[kTcSmall] > 0) {
     auto *kern = &tc_wave_tables<kSmallLogp, kWaves, ITEMS, STREAM>;
     const int grid = grid_for(reinterpret_cast<const void *>(kern),
                               kWaves * 64, 0, l.cus, l.grid_cap,

@@ -31,7 +31,7 @@ class Tune(C.Structure):
                 ("group_work", C.c_int32), ("tc_items", C.c_int32),
                 ("tc_cursor_min", C.c_int32), ("pattern_split", C.c_int32),
                 ("pattern_census", C.c_int32), ("tc_layout", C.c_int32),
-                ("tc_merge_max", C.c_int32)]
+                ("tc_merge_max", C.c_int32), ("tc_half_waves", C.c_int32)]
 
 
 class Stats(C.Structure):

@@ -28,6 +28,11 @@ FORCE = {
     "items8": dict(tc_items=8),
     "items8_global": dict(tc_items=8, small_max_deg=-1, wave_max_deg=-1,
                           block_max_deg=-1),
+    "half": dict(tc_half_waves=1),
+    "half_items8": dict(tc_half_waves=1, tc_items=8),
+    "half_small_only": dict(tc_half_waves=1, small_max_deg=128,
+                            wave_max_deg=-1, block_max_deg=-1),
+    "half_tiny_chunks": dict(tc_half_waves=1, chunk_work=7),
     "merge16": dict(tc_merge_max=16),
     "merge64": dict(tc_merge_max=64),
     "merge64_items4": dict(tc_merge_max=64, tc_items=4),
@@ -101,10 +106,14 @@ def test_tc_er(gpu_available):
     rp, ci = g.csr()
     want = oracle.tc(rp, ci)
     assert _gpu_tc(g)[0] == want
-    # every owner small: all tasks merged, from both list sources
+    # every owner small: all tasks merged, from both list sources; and the
+    # small class two tasks per wave
     for layout in (-1, 1):
         assert _gpu_tc(g, ma.make_tune(tc_merge_max=64,
                                        tc_layout=layout))[0] == want
+    for items in (4, 8):
+        assert _gpu_tc(g, ma.make_tune(tc_half_waves=1, tc_layout=1,
+                                       tc_items=items))[0] == want
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 40])
@@ -131,7 +140,7 @@ def test_tc_dense_hub_uses_big_tables(gpu_available):
                                 for b in range(a + 1, n)])
     g = ma.Graph.from_csr(rp, ci)
     want = n * (n - 1) * (n - 2) // 6
-    for mode in ("default", "global_only", "tiny_chunks", "merge64"):
+    for mode in ("default", "global_only", "tiny_chunks", "merge64", "half"):
         for layout in (-1, 1):
             assert _gpu_tc(g, _tune(mode, layout))[0] == want, (mode, layout)
 
