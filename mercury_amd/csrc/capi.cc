@@ -600,10 +600,20 @@ int mgpm_tc_count(mgpm_dgraph *dg, const mgpm_tune *tune, void *stream,
   if (use_layout) {
     const auto tp = std::chrono::steady_clock::now();
     const bool had = dg->tc.laid.p != nullptr;
-    if ((rc = tc_layout_build(dg, s))) return rc;
-    if (!had) prep += ms_since(tp);
-    plan.items = dg->tc.laid.as<int32_t>();
-    plan.item_off = dg->tc.laid_off.as<uint64_t>();
+    rc = tc_layout_build(dg, s);
+    if (rc == MGPM_ERR_NOMEM && lay == 0) {
+      // By default a layout that cannot be allocated is skipped: the
+      // lists are gathered instead (the failed allocation's error is
+      // cleared so that the launch check does not see it).
+      (void)hipGetLastError();
+      use_layout = false;
+    } else if (rc) {
+      return rc;
+    } else {
+      if (!had) prep += ms_since(tp);
+      plan.items = dg->tc.laid.as<int32_t>();
+      plan.item_off = dg->tc.laid_off.as<uint64_t>();
+    }
   }
   MGPM_HIP(hipEventRecord(dg->ev0, s));
   if (tc_device_launch(plan, l, s))
