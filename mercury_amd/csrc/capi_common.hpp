@@ -153,13 +153,16 @@ struct CensusGraph {
   bool ready = false;
   int32_t max_deg = 1;   // largest degree
   int64_t max_out = 1;   // largest DAG out-degree
+  // The graph relabelled in DAG order (census_relabel: ids are ranks by
+  // degree, then id), its CSR and DAG; every census pass runs on it.
+  DevBuf rowptr, col;
   DevBuf deg, src, drp, dcol, dsrc;
   bool pairs_ready = false;
   uint64_t tab_total = 0;  // pair-table slots
   DevBuf tab_off;          // n + 1 slot offsets
   size_t bytes() const {
-    return deg.bytes + src.bytes + drp.bytes + dcol.bytes + dsrc.bytes +
-           tab_off.bytes;
+    return rowptr.bytes + col.bytes + deg.bytes + src.bytes + drp.bytes +
+           dcol.bytes + dsrc.bytes + tab_off.bytes;
   }
 };
 

@@ -146,7 +146,8 @@ Need need_for(const mgpm_dgraph *dg, int passes, size_t budget) {
   return k;
 }
 
-// Builds the census DAG of the graph on the device, once.
+// Builds the census graph on the device, once: the graph relabelled in DAG
+// order, then its DAG.
 int graph_ready(mgpm_dgraph *dg, hipStream_t s) {
   CensusGraph &cg = dg->census;
   if (cg.ready) return MGPM_OK;
@@ -157,6 +158,20 @@ int graph_ready(mgpm_dgraph *dg, hipStream_t s) {
   if (nnz >= (int64_t(1) << 32))
     return fail(MGPM_ERR_CAPACITY, "census: 2^32 or more arcs");
   CensusGraph g;
+  MGPM_HIP(g.rowptr.alloc((static_cast<size_t>(n) + 1) * sizeof(int64_t)));
+  MGPM_HIP(g.col.alloc(std::max<int64_t>(nnz, 1) * sizeof(int32_t)));
+  {
+    const size_t rb = census_relabel_bytes(n, nnz);
+    if (rb == 0) return fail(MGPM_ERR_HIP, "sort size query");
+    DevBuf rtmp;
+    MGPM_HIP(rtmp.alloc(rb));
+    if (census_relabel(dg->csr.rowptr.as<int64_t>(),
+                       dg->csr.colidx.as<int32_t>(), n, nnz,
+                       g.rowptr.as<int64_t>(), g.col.as<int32_t>(), rtmp.p,
+                       rb, dg->cus, s))
+      return fail(MGPM_ERR_HIP, "census relabelling failed");
+    MGPM_HIP(hipStreamSynchronize(s));  // rtmp is freed here
+  }
   DevBuf nd, val, pos, scan;
   const size_t a1 = static_cast<size_t>(nnz) + 1;
   MGPM_HIP(g.deg.alloc(std::max<int32_t>(n, 1) * sizeof(int32_t)));
@@ -171,8 +186,8 @@ int graph_ready(mgpm_dgraph *dg, hipStream_t s) {
   if (dag_scan_bytes(nnz, &tb)) return fail(MGPM_ERR_HIP, "scan size query");
   MGPM_HIP(scan.alloc(std::max<size_t>(tb, 1)));
   DagDevice d;
-  d.rowptr = dg->csr.rowptr.as<int64_t>();
-  d.col = dg->csr.colidx.as<int32_t>();
+  d.rowptr = g.rowptr.as<int64_t>();
+  d.col = g.col.as<int32_t>();
   d.n = n;
   d.nnz = nnz;
   d.division = false;
@@ -194,6 +209,8 @@ int graph_ready(mgpm_dgraph *dg, hipStream_t s) {
   // The host plan knows the largest DAG out-degree (the same DAG).
   g.max_out = std::max<int64_t>(dg->tc.max_out, 1);
   MGPM_HIP(hipStreamSynchronize(s));  // the work arrays die here
+  cg.rowptr.swap(g.rowptr);
+  cg.col.swap(g.col);
   cg.deg.swap(g.deg);
   cg.src.swap(g.src);
   cg.drp.swap(g.drp);
@@ -222,8 +239,8 @@ CensusArgs args_of(mgpm_dgraph *dg, int passes, CensusWork &w,
   c.slab_len = cg.max_out;
   c.cursor = cursor_of(w);
   c.acc = census_acc(w);
-  c.rowptr = dg->csr.rowptr.as<int64_t>();
-  c.col = dg->csr.colidx.as<int32_t>();
+  c.rowptr = cg.rowptr.as<int64_t>();
+  c.col = cg.col.as<int32_t>();
   c.wedges = w.wedges.as<uint32_t>();
   c.touched = w.touched.as<int32_t>();
   c.c4_groups = k.c4_groups;
@@ -240,8 +257,8 @@ CensusArgs args_of(mgpm_dgraph *dg, int passes, CensusWork &w,
 Census5Args args5_of(mgpm_dgraph *dg, CensusWork &w, const Need &k) {
   const CensusGraph &cg = dg->census;
   Census5Args c;
-  c.rowptr = dg->csr.rowptr.as<int64_t>();
-  c.col = dg->csr.colidx.as<int32_t>();
+  c.rowptr = cg.rowptr.as<int64_t>();
+  c.col = cg.col.as<int32_t>();
   c.src = cg.src.as<int32_t>();
   c.deg = cg.deg.as<int32_t>();
   c.n = dg->host->n;
@@ -336,6 +353,7 @@ int pairs_layout(mgpm_dgraph *dg, hipStream_t s) {
   CensusGraph &cg = dg->census;
   if (cg.pairs_ready) return MGPM_OK;
   const int32_t n = dg->host->n;
+  const int64_t nnz = dg->host->nnz;
   const size_t n1 = static_cast<size_t>(n) + 1;
   const int groups = static_cast<int>(std::max<size_t>(
       1, std::min<size_t>(size_t(dg->cus) * 2,
@@ -351,11 +369,11 @@ int pairs_layout(mgpm_dgraph *dg, hipStream_t s) {
   MGPM_HIP(small.alloc(kSmallBytes));
   MGPM_HIP(scan.alloc(std::max<size_t>(sb, 1)));
   Census5Args c;
-  c.rowptr = dg->csr.rowptr.as<int64_t>();
-  c.col = dg->csr.colidx.as<int32_t>();
+  c.rowptr = cg.rowptr.as<int64_t>();
+  c.col = cg.col.as<int32_t>();
   c.deg = cg.deg.as<int32_t>();
   c.n = n;
-  c.nnz = dg->host->nnz;
+  c.nnz = nnz;
   c.drp = cg.drp.as<int64_t>();
   c.dcol = cg.dcol.as<int32_t>();
   c.stamp = stamp.as<uint32_t>();

@@ -50,6 +50,9 @@ enum CensusAcc : int {
   kCensusSlots = 32
 };
 
+// The census runs on the graph relabelled in DAG order (census_relabel):
+// rank is id, so "ranked below v" is "id below v", every CSR row is sorted
+// with the vertices below its own first, and the DAG rows are the tails.
 struct CensusArgs {
   const int64_t *drp = nullptr;  // DAG row offsets (n + 1)
   const int32_t *dcol = nullptr; // DAG column array (m)
@@ -62,7 +65,8 @@ struct CensusArgs {
   int64_t slab_len = 0;          // >= the largest DAG out-degree
   unsigned *cursor = nullptr;    // arc cursor, then vertex cursor (zeroed)
   unsigned long long *acc = nullptr;  // kCensusSlots (zeroed)
-  // 4-cycle pass (census_c4_launch): undirected CSR, and per workgroup a
+  // 4-cycle pass (census_c4_launch): undirected CSR (relabelled, rows
+  // sorted), and per workgroup a
   // dense counter array (zeroed; left zeroed) and a list, n entries each.
   const int64_t *rowptr = nullptr;
   const int32_t *col = nullptr;
@@ -136,6 +140,19 @@ struct Census5Args {
 // Enqueues t_csr and the per-vertex sums (kCensusTriS, kCensusTv2); needs
 // t and tv from census_launch.
 int census5_flat_launch(const Census5Args &c, int cus, hipStream_t s);
+// The graph relabelled in DAG order: vertex v becomes its rank in the
+// order by (degree, id), so that DAG order is id order -- row v of
+// (out_rowptr, out_col) is sorted, its first d(v) - d+(v) entries lie
+// below v and the rest is N+(v).  Every census sum is a count of subgraphs,
+// the same on either labelling.  One stable sort of the vertices by degree
+// gives the ranks, one sort of the arcs by (rank of the row, rank of the
+// head) the rows.  out_rowptr holds n + 1, out_col nnz words; tmp holds
+// census_relabel_bytes(n, nnz) bytes.  Returns 0, or -1 if a launch, scan
+// or sort failed.
+size_t census_relabel_bytes(int32_t n, int64_t nnz);
+int census_relabel(const int64_t *rowptr, const int32_t *col, int32_t n,
+                   int64_t nnz, int64_t *out_rowptr, int32_t *out_col,
+                   void *tmp, size_t tmp_bytes, int cus, hipStream_t s);
 // Enqueues the pair-table count pass (tab_cap, from zeroed stamps); the
 // caller scans tab_cap into tab_off and sizes the table from tab_off[n].
 int census5_table_count_launch(const Census5Args &c, hipStream_t s);

@@ -21,9 +21,13 @@
 //     is the hinge end x (tips p, q and hinge end y in N+(x)) give
 //     |N+(x) ∩ N(p) ∩ N(q)| (codeg(p, q) - 2) of K2,3+e, codeg from the
 //     table (x lies below p and q, so the pair is there).
-// G[N+(a)] is held as adjacency bit rows, in LDS for d+ up to 512.
+// G[N+(a)] is held as adjacency bit rows, in LDS for d+ up to 512.  The
+// passes run on the census graph, relabelled in DAG order (census_relabel):
+// ids compare as ranks, and each sorted row lists the vertices below its
+// own first, then N+.
 #include <hip/hip_runtime.h>
 
+#include <hipcub/device/device_radix_sort.hpp>
 #include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
@@ -39,12 +43,9 @@ namespace {
 constexpr int kT = 256;  // threads per workgroup
 constexpr int kTW = kT / 64;
 
-__device__ __forceinline__ bool above(const int32_t *__restrict__ deg,
-                                      int32_t a, int32_t b) {
-  // b is ranked above a
-  const int32_t da = deg[a], db = deg[b];
-  return db > da || (db == da && b > a);
-}
+// b is ranked above a in DAG order: the census graph is relabelled in that
+// order (census_relabel), so the larger id.
+__device__ __forceinline__ bool above(int32_t a, int32_t b) { return b > a; }
 
 template <class P>
 __device__ __forceinline__ int lower_bound_p(P list, int len, int32_t key) {
@@ -119,7 +120,7 @@ __global__ void c5_t_csr(Census5Args c) {
   for (int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < c.nnz;
        j += int64_t(gridDim.x) * blockDim.x) {
     const int32_t v = c.src[j], u = c.col[j];
-    const bool vu = above(c.deg, v, u);
+    const bool vu = above(v, u);
     const int32_t lo = vu ? v : u, hi = vu ? u : v;
     const int64_t b = c.drp[lo];
     const int len = static_cast<int>(c.drp[lo + 1] - b);
@@ -211,15 +212,15 @@ __global__ __launch_bounds__(kT) void c5_tab_count(Census5Args c) {
     if (ua >= static_cast<unsigned>(c.n)) break;
     const int32_t a = static_cast<int32_t>(ua);
     const uint32_t tag = ua + 1u;
-    const int64_t ea = c.rowptr[a + 1];
+    // z below a: the first d(a) - d+(a) entries of a's row in DAG order.
+    const int64_t ea = c.rowptr[a + 1] - (c.drp[a + 1] - c.drp[a]);
     for (int64_t k = c.rowptr[a] + wv; k < ea; k += kTW) {
       const int32_t z = c.col[k];
-      if (!above(c.deg, z, a)) continue;  // z must lie below a
       const int64_t bz = c.drp[z];
       const int dz = static_cast<int>(c.drp[z + 1] - bz);
       for (int q = lane; q < dz; q += 64) {
         const int32_t cc = c.dcol[bz + q];
-        if (!above(c.deg, a, cc)) continue;
+        if (!above(a, cc)) continue;
         if (atomicExch(&stamp[cc], tag) != tag) atomicAdd(&s_cnt, 1u);
       }
     }
@@ -316,10 +317,12 @@ __global__ __launch_bounds__(kT) void c5_anchor(Census5Args c, int cls) {
       const int32_t y = W > 0 ? s_nb[i] : gnb[i];
       const unsigned long long bit = 1ull << (i & 63);
       const int wd = i >> 6;
+      // c above a: the tail of y's sorted row past a (a itself lies in the
+      // row, y being above a).
       const int64_t ey = c.rowptr[y + 1];
-      for (int64_t k = c.rowptr[y] + lane; k < ey; k += 64) {
+      const int64_t ky = wave_first_greater(c.col, a, c.rowptr[y], ey);
+      for (int64_t k = ky + lane; k < ey; k += 64) {
         const int32_t cc = c.col[k];
-        if (!above(c.deg, a, cc)) continue;
         const unsigned long long old =
             atomicOr(&mask[size_t(cc) * rs + wd], bit);
         // One-word masks: the first bit set on c (its mask and count were
@@ -329,16 +332,16 @@ __global__ __launch_bounds__(kT) void c5_anchor(Census5Args c, int cls) {
       }
     }
     if (W == 1) __syncthreads();  // every mask final before the counts
-    // lo: common neighbours below a, through every z below a.
-    const int64_t ea = c.rowptr[a + 1];
+    // lo: common neighbours below a, through every z below a (the first
+    // d(a) - d+(a) entries of a's row in DAG order).
+    const int64_t ea = c.rowptr[a + 1] - dp;
     for (int64_t k = c.rowptr[a] + wv; k < ea; k += kTW) {
       const int32_t z = c.col[k];
-      if (!above(c.deg, z, a)) continue;
       const int64_t bz = c.drp[z];
       const int dz = static_cast<int>(c.drp[z + 1] - bz);
       for (int q = lane; q < dz; q += 64) {
         const int32_t cc = c.dcol[bz + q];
-        if (!above(c.deg, a, cc)) continue;
+        if (!above(a, cc)) continue;
         const uint32_t old = atomicAdd(&lo[cc], 1u);
         bool first;
         if (W == 1)  // listed already if its mask has a bit
@@ -475,7 +478,7 @@ __global__ __launch_bounds__(kT) void c5_outnb(Census5Args c, int cls) {
         cyc += choose2(h);
         const int32_t p = W > 0 ? s_nb[i] : gnb[i];
         const int32_t q = W > 0 ? s_nb[j] : gnb[j];
-        const bool pq = above(c.deg, p, q);
+        const bool pq = above(p, q);
         const int32_t lo_v = pq ? p : q, hi_v = pq ? q : p;
         const uint64_t off = c.tab_off[lo_v];
         uint32_t cd = 0;
@@ -497,6 +500,108 @@ __global__ __launch_bounds__(kT) void c5_outnb(Census5Args c, int cls) {
     if (miss) atomicAdd(&c.acc[kCensusTableMiss], miss);
   }
   if (threadIdx.x == 0 && w4) atomicAdd(&c.acc[kCensusW4], w4);
+}
+
+// Relabelling in DAG order (census_relabel): degrees and ids, then ranks.
+__global__ void c5_degrees(const int64_t *__restrict__ rowptr, int32_t n,
+                           uint32_t *__restrict__ deg,
+                           uint32_t *__restrict__ ids) {
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+       v += int64_t(gridDim.x) * blockDim.x) {
+    deg[v] = static_cast<uint32_t>(rowptr[v + 1] - rowptr[v]);
+    ids[v] = static_cast<uint32_t>(v);
+  }
+}
+
+// rank[perm[i]] = i: perm lists the vertices in DAG order; lens[i] = the
+// degree of rank i (lens[n] = 0, for the scan into row pointers).
+__global__ void c5_ranks(const uint32_t *__restrict__ perm,
+                         const uint32_t *__restrict__ dsort, int32_t n,
+                         uint32_t *__restrict__ rank,
+                         int64_t *__restrict__ lens) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i <= n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    if (i < n) rank[perm[i]] = static_cast<uint32_t>(i);
+    lens[i] = i < n ? static_cast<int64_t>(dsort[i]) : 0;
+  }
+}
+
+// One key per arc, one wave per row: the rank of its row above the rank
+// of its head (lg bits each).
+__global__ void c5_arc_keys(const int64_t *__restrict__ rowptr,
+                            const int32_t *__restrict__ col, int32_t n,
+                            const uint32_t *__restrict__ rank, int lg,
+                            unsigned long long *__restrict__ keys) {
+  const int lane = lane64();
+  const int64_t nwaves = (int64_t(gridDim.x) * blockDim.x) >> 6;
+  for (int64_t v = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+       v < n; v += nwaves) {
+    const unsigned long long r = static_cast<unsigned long long>(rank[v])
+                                 << lg;
+    const int64_t e = rowptr[v + 1];
+    for (int64_t i = rowptr[v] + lane; i < e; i += 64)
+      keys[i] = r | rank[col[i]];
+  }
+}
+
+// The heads' new ids (their ranks), in the sorted order.
+__global__ void c5_arc_heads(const unsigned long long *__restrict__ keys,
+                             int64_t nnz, unsigned long long low,
+                             int32_t *__restrict__ out) {
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < nnz;
+       e += int64_t(gridDim.x) * blockDim.x)
+    out[e] = static_cast<int32_t>(keys[e] & low);
+}
+
+// Layout of census_relabel's scratch: degrees, ids, sorted degrees, the
+// order and the ranks (n words each), the row lengths (n + 1 int64), the
+// arc keys and their sorted copy (nnz u64 each), and the storage of the
+// sorts and the scan.
+struct RelabelTmp {
+  size_t deg = 0, ids = 0, dsort = 0, perm = 0, rank = 0, lens = 0, keys = 0,
+         keys2 = 0, work = 0, work_bytes = 0, total = 0;
+  int lg = 1;  // bits of a vertex id
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+bool relabel_tmp(int32_t n, int64_t nnz, RelabelTmp *t) {
+  const size_t nv = static_cast<size_t>(std::max<int32_t>(n, 1));
+  const size_t na = static_cast<size_t>(std::max<int64_t>(nnz, 1));
+  t->lg = 1;
+  while (t->lg < 31 && (int64_t(1) << t->lg) < int64_t(n)) ++t->lg;
+  size_t b1 = 0, b2 = 0, b3 = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(
+          nullptr, b1, static_cast<const uint32_t *>(nullptr),
+          static_cast<uint32_t *>(nullptr),
+          static_cast<const uint32_t *>(nullptr),
+          static_cast<uint32_t *>(nullptr), static_cast<int64_t>(nv), 0,
+          32) != hipSuccess)
+    return false;
+  if (hipcub::DeviceRadixSort::SortKeys(
+          nullptr, b2, static_cast<const unsigned long long *>(nullptr),
+          static_cast<unsigned long long *>(nullptr),
+          static_cast<int64_t>(na), 0, 2 * t->lg) != hipSuccess)
+    return false;
+  if (hipcub::DeviceScan::ExclusiveSum(
+          nullptr, b3, static_cast<const int64_t *>(nullptr),
+          static_cast<int64_t *>(nullptr),
+          static_cast<int64_t>(nv + 1)) != hipSuccess)
+    return false;
+  size_t o = 0;
+  t->deg = o;   o += align256(nv * 4);
+  t->ids = o;   o += align256(nv * 4);
+  t->dsort = o; o += align256(nv * 4);
+  t->perm = o;  o += align256(nv * 4);
+  t->rank = o;  o += align256(nv * 4);
+  t->lens = o;  o += align256((nv + 1) * 8);
+  t->keys = o;  o += align256(na * 8);
+  t->keys2 = o; o += align256(na * 8);
+  t->work = o;
+  t->work_bytes = std::max<size_t>(std::max({b1, b2, b3}), 1);
+  o += align256(t->work_bytes);
+  t->total = o;
+  return true;
 }
 
 // Workgroups of a class whose masks take `words` words per vertex: as many
@@ -550,6 +655,60 @@ int census5_scan(const unsigned long long *in, unsigned long long *out,
                  hipSuccess
              ? 0
              : -1;
+}
+
+size_t census_relabel_bytes(int32_t n, int64_t nnz) {
+  RelabelTmp t;
+  return relabel_tmp(n, nnz, &t) ? t.total : 0;
+}
+
+int census_relabel(const int64_t *rowptr, const int32_t *col, int32_t n,
+                   int64_t nnz, int64_t *out_rowptr, int32_t *out_col,
+                   void *tmp, size_t tmp_bytes, int cus, hipStream_t s) {
+  if (n < 0) return -1;
+  RelabelTmp t;
+  if (!relabel_tmp(n, nnz, &t) || tmp_bytes < t.total) return -1;
+  char *base = static_cast<char *>(tmp);
+  auto *deg = reinterpret_cast<uint32_t *>(base + t.deg);
+  auto *ids = reinterpret_cast<uint32_t *>(base + t.ids);
+  auto *dsort = reinterpret_cast<uint32_t *>(base + t.dsort);
+  auto *perm = reinterpret_cast<uint32_t *>(base + t.perm);
+  auto *rank = reinterpret_cast<uint32_t *>(base + t.rank);
+  auto *lens = reinterpret_cast<int64_t *>(base + t.lens);
+  auto *keys = reinterpret_cast<unsigned long long *>(base + t.keys);
+  auto *keys2 = reinterpret_cast<unsigned long long *>(base + t.keys2);
+  void *work = base + t.work;
+  const int gv = static_cast<int>(std::min<int64_t>(4096, n / 256 + 1));
+  size_t b = t.work_bytes;
+  if (n > 0) {
+    hipLaunchKernelGGL(c5_degrees, dim3(gv), dim3(256), 0, s, rowptr, n, deg,
+                       ids);
+    // Radix sorts are stable: equal degrees keep id order (DAG order).
+    if (hipcub::DeviceRadixSort::SortPairs(work, b, deg, dsort, ids, perm,
+                                           static_cast<int64_t>(n), 0, 32,
+                                           s) != hipSuccess)
+      return -1;
+  }
+  hipLaunchKernelGGL(c5_ranks, dim3(gv), dim3(256), 0, s, perm, dsort, n,
+                     rank, lens);
+  b = t.work_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(work, b, lens, out_rowptr,
+                                       static_cast<int64_t>(n) + 1,
+                                       s) != hipSuccess)
+    return -1;
+  if (n > 0 && nnz > 0) {
+    hipLaunchKernelGGL(c5_arc_keys, dim3(std::max(1, cus * 8)), dim3(256), 0,
+                       s, rowptr, col, n, rank, t.lg, keys);
+    b = t.work_bytes;
+    if (hipcub::DeviceRadixSort::SortKeys(work, b, keys, keys2, nnz, 0,
+                                          2 * t.lg, s) != hipSuccess)
+      return -1;
+    const int ga =
+        static_cast<int>(std::min<int64_t>(8192, nnz / 256 + 1));
+    hipLaunchKernelGGL(c5_arc_heads, dim3(ga), dim3(256), 0, s, keys2, nnz,
+                       (1ull << t.lg) - 1ull, out_col);
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int census5_pairs_launch(const Census5Args &c, hipStream_t s) {

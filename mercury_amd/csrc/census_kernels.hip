@@ -648,12 +648,6 @@ __global__ __launch_bounds__(kWaves * 64) void census_k5(CensusArgs c) {
 // sum_w C(wedges(w), 2).  One workgroup per v (vertices handed out by an
 // atomic cursor), with its own dense counter array over all vertices
 // (left zeroed) and the list of the w it touched.
-__device__ __forceinline__ bool rank_below(const int32_t *__restrict__ deg,
-                                           int32_t a, int32_t b) {
-  const int32_t da = deg[a], db = deg[b];
-  return da < db || (da == db && a < b);
-}
-
 constexpr int kC4Threads = 256;
 
 __global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
@@ -673,14 +667,17 @@ __global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
     __syncthreads();
     const unsigned v = s_v;
     if (v >= static_cast<unsigned>(c.n)) break;
-    const int64_t ev = c.rowptr[v + 1];
+    // u below v: the first d(v) - d+(v) entries of v's sorted row; w below
+    // v: the entries of u's row before v (the graph is relabelled in DAG
+    // order, so rank is id).
+    const int64_t ev = c.rowptr[v + 1] - (c.drp[v + 1] - c.drp[v]);
     for (int64_t i = c.rowptr[v] + wv; i < ev; i += nwv) {
       const int32_t u = c.col[i];
-      if (!rank_below(c.deg, u, static_cast<int32_t>(v))) continue;
-      const int64_t eu = c.rowptr[u + 1];
-      for (int64_t j = c.rowptr[u] + lane; j < eu; j += 64) {
+      const int64_t bu = c.rowptr[u];
+      const int64_t eu = wave_first_greater(
+          c.col, static_cast<int32_t>(v) - 1, bu, c.rowptr[u + 1]);
+      for (int64_t j = bu + lane; j < eu; j += 64) {
         const int32_t w = c.col[j];
-        if (!rank_below(c.deg, w, static_cast<int32_t>(v))) continue;
         if (atomicAdd(&cnt[w], 1u) == 0u) touched[atomicAdd(&s_nt, 1u)] = w;
       }
     }
@@ -692,13 +689,13 @@ __global__ __launch_bounds__(kC4Threads) void census_c4(CensusArgs c) {
       // per vertex, weighted by t_e and d_u - 2).
       for (int64_t i = c.rowptr[v] + wv; i < ev; i += nwv) {
         const int32_t u = c.col[i];
-        if (!rank_below(c.deg, u, static_cast<int32_t>(v))) continue;
         const unsigned long long tvu = c.t_csr[i];
         const unsigned long long du2 = c.deg[u] - 2;
-        const int64_t eu = c.rowptr[u + 1];
-        for (int64_t j = c.rowptr[u] + lane; j < eu; j += 64) {
+        const int64_t bu = c.rowptr[u];
+        const int64_t eu = wave_first_greater(
+            c.col, static_cast<int32_t>(v) - 1, bu, c.rowptr[u + 1]);
+        for (int64_t j = bu + lane; j < eu; j += 64) {
           const int32_t w = c.col[j];
-          if (!rank_below(c.deg, w, static_cast<int32_t>(v))) continue;
           // The counts were made by atomics (in L2): read them past L1.
           const unsigned long long x =
               __hip_atomic_load(&cnt[w], __ATOMIC_RELAXED,

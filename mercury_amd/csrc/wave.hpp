@@ -93,6 +93,29 @@ __device__ __forceinline__ uint32_t slot_of(int32_t key, int logp) {
   return (static_cast<uint32_t>(key) * 0x9E3779B1u) >> (32 - logp);
 }
 
+// First position in [lo, hi) of a sorted int32 row whose value exceeds
+// key (hi if none), found by the whole wave: 64 evenly spaced pivots per
+// round narrow the range to one gap, then its last (at most 64) positions
+// are tested at once.  Every lane of the wave must call it with the same
+// arguments; every lane returns it.
+__device__ inline int64_t wave_first_greater(const int32_t *__restrict__ row,
+                                             int32_t key, int64_t lo,
+                                             int64_t hi) {
+  const int lane = lane64();
+  while (hi - lo > 64) {
+    const int64_t span = hi - lo;  // pivot j at lo + span (j + 1) / 65
+    const int64_t p = lo + span * (lane + 1) / 65;
+    const unsigned long long up = __ballot(row[p] > key);
+    const int f = up ? __builtin_ctzll(up) : 64;  // first pivot above key
+    const int64_t nlo = f == 0 ? lo : lo + span * f / 65 + 1;
+    if (f < 64) hi = lo + span * (f + 1) / 65;
+    lo = nlo;
+  }
+  const int64_t p = lo + lane;
+  const unsigned long long up = __ballot(p < hi && row[p] > key);
+  return up ? lo + __builtin_ctzll(up) : hi;
+}
+
 // ceil(log2(x)) for x >= 2.
 __device__ __forceinline__ int ceil_log2(int x) { return 32 - __clz(x - 1); }
 
