@@ -211,6 +211,8 @@ __global__ __launch_bounds__(WAVES * 64) void tc_global_search(
   flush_hits(acc, result);
 }
 
+constexpr int kOwnPerLane = kTcSmallMaxDeg / 32;  // owner items per lane
+
 // Small-class tasks on laid-out lists, two per wave: lanes 0-31 take the
 // group's tasks t0, t0 + 2, ... and lanes 32-63 t0 + 1, t0 + 3, ..., each
 // half with its own table (up to 256 slots) and its own items, 32 x ITEMS
@@ -236,43 +238,65 @@ __global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
     g = wave_first(g);
     if (g >= static_cast<uint32_t>(ngroups)) break;
     const int t0 = goff[g], t_end = goff[g + 1];
-    // The next pair's record, item offsets and first owner item are loaded
-    // one pair ahead (an idle half reads the other half's task).
+    // Loads run ahead so that a pair waits about one memory round trip:
+    // each pair issues the record two pairs ahead, the next pair's item
+    // offsets (its record came in during the previous pair), its own owner
+    // items and its first step of streamed items together, at clamped
+    // (valid) addresses, before it clears and fills its table.  An idle
+    // half reads the other half's task.
     auto rec_of = [&](int t) {
       const int tt = t + sub;
       return tasks[tt < t_end ? tt : (t < t_end ? t : t_end - 1)];
     };
     TcTask task = rec_of(t0);
+    TcTask next = rec_of(t0 + 2);
     uint64_t ib = item_off[task.pbeg];
     uint64_t ie = item_off[task.pbeg + static_cast<uint32_t>(task.pcnt)];
     for (int t = t0; t < t_end; t += 2) {
       const bool have = t + sub < t_end;
-      const TcTask next = rec_of(t + 2);
+      const TcTask next2 = rec_of(t + 4);
       const uint64_t nib = item_off[next.pbeg];
       const uint64_t nie =
           item_off[next.pbeg + static_cast<uint32_t>(next.pcnt)];
-      int logp = ceil_log2(8 * task.tlen);
+      const int tl = task.tlen;  // 1 .. kTcSmallMaxDeg
+      int32_t own[kOwnPerLane];
+#pragma unroll
+      for (int j = 0; j < kOwnPerLane; ++j) {
+        const int k = gl + 32 * j;
+        own[j] = col[task.tbeg + static_cast<uint32_t>(k < tl ? k : 0)];
+      }
+      const uint64_t n = have ? ie - ib : 0;
+      int32_t w[ITEMS];
+      bool ok[ITEMS];
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        const uint64_t k = uint64_t(32 * i + gl);
+        ok[i] = k < n;
+        const int32_t v = items[ib + (ok[i] ? k : 0)];  // clamped: ib < ie
+        w[i] = ok[i] ? v : -2;
+      }
+      int logp = ceil_log2(8 * tl);
       logp = logp < 5 ? 5 : (logp > kSmallLogpHalf ? kSmallLogpHalf : logp);
       const int P = 1 << logp;
       for (int i = gl; i < P; i += 32) table[i] = -1;
       wave_sync();
       if (have) {
         const uint32_t mask = static_cast<uint32_t>(P - 1);
-        for (int k = gl; k < task.tlen; k += 32) {
-          const int32_t key = col[task.tbeg + static_cast<uint32_t>(k)];
+#pragma unroll
+        for (int j = 0; j < kOwnPerLane; ++j) {
+          if (gl + 32 * j >= tl) continue;
+          const int32_t key = own[j];
           uint32_t h = slot_of(key, logp);
           while (atomicCAS(&table[h], -1, key) != -1) h = (h + 1) & mask;
         }
       }
       wave_sync();
       HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
-      const uint64_t n = have ? ie - ib : 0;
+      acc += probe.template lookup<ITEMS>(w, ok);  // the first step
       const uint64_t steps = (n + 32 * ITEMS - 1) / (32 * ITEMS);
       const uint64_t other = __shfl_xor(steps, 32, 64);
       const uint64_t wsteps = steps > other ? steps : other;
-      for (uint64_t st = 0; st < wsteps; ++st) {
-        int32_t w[ITEMS];
-        bool ok[ITEMS];
+      for (uint64_t st = 1; st < wsteps; ++st) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
           const uint64_t k = st * (32 * ITEMS) + uint64_t(32 * i + gl);
@@ -285,6 +309,7 @@ __global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
       }
       wave_sync();  // the tables are cleared for the next pair
       task = next;
+      next = next2;
       ib = nib;
       ie = nie;
     }
