@@ -241,9 +241,9 @@ __global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
     // Loads run ahead so that a pair waits about one memory round trip:
     // each pair issues the record two pairs ahead, the next pair's item
     // offsets (its record came in during the previous pair), its own owner
-    // items and its first step of streamed items together, at clamped
-    // (valid) addresses, before it clears and fills its table.  An idle
-    // half reads the other half's task.
+    // items and its first two steps of streamed items together, at
+    // clamped (valid) addresses, before it clears and fills its table.  An
+    // idle half reads the other half's task.
     auto rec_of = [&](int t) {
       const int tt = t + sub;
       return tasks[tt < t_end ? tt : (t < t_end ? t : t_end - 1)];
@@ -266,14 +266,17 @@ __global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
         own[j] = col[task.tbeg + static_cast<uint32_t>(k < tl ? k : 0)];
       }
       const uint64_t n = have ? ie - ib : 0;
-      int32_t w[ITEMS];
-      bool ok[ITEMS];
+      int32_t w[ITEMS], w1[ITEMS];
+      bool ok[ITEMS], ok1[ITEMS];
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) {
+      for (int i = 0; i < ITEMS; ++i) {  // the first two steps
         const uint64_t k = uint64_t(32 * i + gl);
         ok[i] = k < n;
+        ok1[i] = k + 32 * ITEMS < n;
         const int32_t v = items[ib + (ok[i] ? k : 0)];  // clamped: ib < ie
+        const int32_t v1 = items[ib + (ok1[i] ? k + 32 * ITEMS : 0)];
         w[i] = ok[i] ? v : -2;
+        w1[i] = ok1[i] ? v1 : -2;
       }
       int logp = ceil_log2(8 * tl);
       logp = logp < 5 ? 5 : (logp > kSmallLogpHalf ? kSmallLogpHalf : logp);
@@ -292,11 +295,12 @@ __global__ __launch_bounds__(WAVES * 64) void tc_half_tables(
       }
       wave_sync();
       HashProbe<false> probe{table, logp, static_cast<uint32_t>(P - 1)};
-      acc += probe.template lookup<ITEMS>(w, ok);  // the first step
+      acc += probe.template lookup<ITEMS>(w, ok);  // the first two steps
+      acc += probe.template lookup<ITEMS>(w1, ok1);
       const uint64_t steps = (n + 32 * ITEMS - 1) / (32 * ITEMS);
       const uint64_t other = __shfl_xor(steps, 32, 64);
       const uint64_t wsteps = steps > other ? steps : other;
-      for (uint64_t st = 1; st < wsteps; ++st) {
+      for (uint64_t st = 2; st < wsteps; ++st) {
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
           const uint64_t k = st * (32 * ITEMS) + uint64_t(32 * i + gl);
