@@ -20,8 +20,8 @@ import csv
 import json
 from pathlib import Path
 
-COUNTING = ("tc_wave_tables", "tc_block_tables", "tc_global_search",
-            "pattern_", "batch_kernel")
+COUNTING = ("tc_wave_tables", "tc_half_tables", "tc_block_tables",
+            "tc_global_search", "pattern_", "batch_kernel")
 
 
 def _short(name: str) -> str:
