@@ -511,8 +511,11 @@ def run_extras(ma, args) -> dict:
         out["cpu_small_baselines"] = _guarded("cpu baselines",
                                               cpu_small_baselines, ma)
     if "batch" in extras:
+        # config 4 as one launch: the batch recorded into a HIP graph (the
+        # timed batch plans, records and launches it; no warm-up batch)
         a = copy.copy(args)
         a.batch_warmup = 0
+        a.batch_use_graph = 1
         a.batch_oracle_ef = min(args.batch_oracle_ef, 8)
         out["batch"] = _guarded("batch", run_batch, ma, a)
     if "serve" in extras:
